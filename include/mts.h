@@ -1,0 +1,166 @@
+/*
+ * C ABI of the MI355X Matched-TS per-time-step path.
+ *
+ * The reference (a single-threaded C++14 program) has no plugin ABI: its hot
+ * path sits behind C++ classes.  Each entry point below replaces one of those
+ * class operations; the citation names the reference interface (paths
+ * relative to the reference's src/).  All functions return 0 on success and a
+ * negative MTS_E_* code otherwise; mts_last_error() gives the message for the
+ * calling thread.  No exception crosses this boundary.  The library owns
+ * device buffers; the caller owns host buffers.  A context is not safe for
+ * concurrent use from several threads.
+ */
+#ifndef MTS_H
+#define MTS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MTS_OK = 0,
+  MTS_E_ARG = -1,       /* invalid argument */
+  MTS_E_GEOMETRY = -2,  /* setup refused: the reference would have exited (message says why) */
+  MTS_E_DEVICE = -3,    /* HIP runtime error */
+  MTS_E_STATE = -4,     /* call out of order */
+  MTS_E_NOMEM = -5
+};
+
+/* temporal methods (constant.h: Temporal_Method_Type) */
+enum { MTS_ADI = 0, MTS_LOD_IE = 1, MTS_LOD_CN = 2, MTS_TS = 3 };
+/* surfaces (surface/ *) */
+enum { MTS_SURF_ELLIPSOID = 0, MTS_SURF_TORUS = 1, MTS_SURF_BUMPY = 2, MTS_SURF_CALLBACK = 3 };
+/* diffusion coefficients: beta- = p0, beta+ = p1 (CONST); beta- = p0 (1 + sin x cos y / 2) (SINCOS) */
+enum { MTS_BETA_CONST = 0, MTS_BETA_SINCOS = 1 };
+/* field layouts for upload/download */
+enum {
+  MTS_LAYOUT_XFAST = 0, /* index (iz*ny + iy)*nx + ix (the reference's To1d, mesh.cc:93-99) */
+  MTS_LAYOUT_REF = 1    /* index (ix*ny + iy)*nz + iz (the reference's CubicDoub uh[ix][iy][iz]) */
+};
+
+typedef struct {
+  double lo[3], hi[3]; /* box */
+  int n[3];            /* node counts per axis (the reference's x/y/z-grid) */
+} mts_grid_desc;
+
+typedef struct {
+  int kind;      /* MTS_SURF_* */
+  double p[8];   /* ellipsoid: a, b, c, r; torus: R, r; bumpy: R0, A, m, k */
+  double (*phi)(void *user, double x, double y, double z);             /* CALLBACK */
+  void (*grad)(void *user, double x, double y, double z, double g[3]); /* CALLBACK */
+  void *user;
+} mts_surface_desc;
+
+typedef struct {
+  int beta_kind; /* MTS_BETA_* */
+  double bp[4];  /* beta parameters */
+} mts_physics_desc;
+
+/* One interface crossing, the live fields of the reference's
+ * Intersection_Data (constant.h:141-156). */
+typedef struct {
+  int dir;          /* 0 = x, 1 = y, 2 = z line */
+  int j, k;         /* the line's fixed indices: x line (iy, iz), y line (ix, iz), z line (ix, iy) */
+  int id;           /* ID: 1-based count along the line, negative for a corner pair */
+  int left_loc;
+  int nw;           /* 6 irregular, 9 corner */
+  double gamma;
+  double coord[3];
+  double normal[3];
+  double p[9];      /* local frame rows (n, eta, tau) */
+  double pinv_d0;   /* P^{-1}[dir][0] */
+  double beta_in, beta_out;
+  double wl[9], wr[9]; /* wei.weil[0], wei.weir[0] */
+} mts_crossing;
+
+typedef struct {
+  int n[3];
+  long long n_cross[3];
+  int corner_pairs[3];
+  int singular_warnings;
+  int mib;
+} mts_setup_info;
+
+typedef struct mts_setup mts_setup;
+typedef struct mts_ctx mts_ctx;
+typedef struct mts_instance mts_instance;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char *mts_last_error(void);
+/* Library version string. */
+const char *mts_version(void);
+
+/* ---------------------------------------------------------------- setup
+ * Mesh + Intersections construction on the host in FP64: node classification
+ * (Mesh::Mesh mesh/mesh.cc:20-70), crossings, normals, local frames and the
+ * MIB-L1/L2 fictitious-point weights (Intersections::Intersections
+ * spatial/intersection.cc:22-51 and its setup chain :79-1830).
+ * mib = 1 (MIB-L1, "mib-v1") or 2 (MIB-L2, "mib-v2"); accuracy = 2. */
+int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
+                     const mts_physics_desc *physics, int mib, int accuracy, mts_setup **out);
+void mts_setup_destroy(mts_setup *s);
+int mts_setup_get_info(const mts_setup *s, mts_setup_info *info);
+/* node classification, +1 outside / -1 inside, MTS_LAYOUT_XFAST (Mesh::mesh_value) */
+int mts_setup_indicator(const mts_setup *s, int8_t *out, size_t count);
+/* node coordinates along axis a (Mesh::xi/yi/zi) */
+int mts_setup_coords(const mts_setup *s, int axis, double *out, size_t count);
+/* crossings of direction dir in line order (Intersections::ifpx/ifpy/ifpz) */
+int mts_setup_crossings(const mts_setup *s, int dir, mts_crossing *out, size_t count);
+
+/* ---------------------------------------------------------------- device */
+int mts_ctx_create(int device, mts_ctx **out);
+void mts_ctx_destroy(mts_ctx *ctx);
+int mts_ctx_device_info(mts_ctx *ctx, char *name, size_t name_len, size_t *hbm_bytes);
+
+/* A device-resident solver for one problem: the constructor of Douglas_ADI
+ * (temporal/douglas_adi.cc:15-53), LOD (temporal/lod.cc) or TS
+ * (temporal/trapezoidal.cc).  dt is the time step; the LHS tables are built
+ * for it. */
+int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                        int method, double dt, mts_instance **out);
+void mts_instance_destroy(mts_instance *inst);
+
+/* Field transfer (host <-> the instance's solution field). */
+int mts_field_upload(mts_instance *inst, const double *host, int layout);
+int mts_field_download(mts_instance *inst, double *host, int layout);
+
+/* Exact solution at t0 on every node (Initialization, douglas_adi.cc:1684-1714). */
+int mts_initialize(mts_instance *inst, double t0);
+
+/* One time step ending at t_next (Solve_2nd: douglas_adi.cc:66-1110,
+ * lod.cc:44-205, trapezoidal.cc:46-272).  TS evaluates its three equations at
+ * t_next - dt, t_next - dt/2 and t_next as the reference's driver does
+ * (helper.h:371-383).  Asynchronous on the instance's stream. */
+int mts_step(mts_instance *inst, double t_next);
+
+/* The reference's time loop (helper.h:199-238, 280-319, 361-401):
+ * steps = (int)(t_finish / dt), t += dt per step, starting from t0.
+ * Returns the number of steps taken in *steps and the final time in *t_end. */
+int mts_run(mts_instance *inst, double t0, double t_finish, int *steps, double *t_end);
+
+/* Wait for the instance's queued work. */
+int mts_sync(mts_instance *inst);
+
+/* L-infinity (with its first argmax in the reference's loop order) and
+ * L2 = sqrt(sum e^2 / (nx ny nz)) of u - exact at time t
+ * (Error, douglas_adi.cc:1723-1770). */
+int mts_error_norms(mts_instance *inst, double t, double *linf, double *l2, int argmax[3]);
+
+/* Device pointer of the solution field (x pitch in *pitch, in doubles) and the
+ * instance's HIP stream, for callers that keep data on the device. */
+int mts_field_device(mts_instance *inst, double **dptr, int *pitch, void **stream);
+
+/* Batched tridiagonal solve (the reference's TDMA, douglas_adi.cc:1784-1807,
+ * for many lines at once): nlines systems of length n, host arrays line-major;
+ * a[l*n] and c[l*n + n - 1] are ignored. */
+int mts_tdma_batched(mts_ctx *ctx, int n, int nlines, const double *a, const double *b,
+                     const double *c, const double *r, double *u);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MTS_H */

@@ -1,0 +1,321 @@
+"""Python binding of the MI355X Matched-TS per-step path (ctypes over libmts.so).
+
+The product is the C ABI in ``include/mts.h`` (HIP kernels for gfx950, host
+FP64 setup, C++ drop-in classes).  This module is a thin ctypes layer used by
+the tests and the benchmark; it mirrors the reference's objects:
+
+* :class:`Setup`     -- ``Mesh`` + ``Intersections`` (reference mesh/mesh.cc,
+  spatial/intersection.cc): node classification, crossings, MIB weights.
+* :class:`Instance`  -- ``Douglas_ADI`` / ``LOD`` / ``TS`` (reference
+  temporal/*.cc): a device-resident solver with ``initialize``, ``step``
+  (``Solve_2nd``), ``run`` (the driver loops of helper.h) and ``error``.
+
+Nothing here computes anything on the CPU: every numerical call goes to the
+library, and the library runs the per-step path on the GPU only.  Loading
+fails loudly when the library has not been built.
+
+The directory name is not a Python identifier, so load it with
+``importlib.util.spec_from_file_location("mts_amd", <dir>/__init__.py)``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libmts.so")
+
+ADI, LOD_IE, LOD_CN, TS = 0, 1, 2, 3
+METHODS = {"adi": ADI, "lod-ie": LOD_IE, "lod-cn": LOD_CN, "ts": TS}
+SURF_ELLIPSOID, SURF_TORUS, SURF_BUMPY, SURF_CALLBACK = 0, 1, 2, 3
+BETA_CONST, BETA_SINCOS = 0, 1
+LAYOUT_XFAST, LAYOUT_REF = 0, 1
+
+
+class MtsError(RuntimeError):
+    """A failed call of the C ABI; ``code`` is the MTS_E_* status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"mts error {code}: {msg}")
+        self.code = code
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("lo", C.c_double * 3), ("hi", C.c_double * 3), ("n", C.c_int * 3)]
+
+
+PHI_FN = C.CFUNCTYPE(C.c_double, C.c_void_p, C.c_double, C.c_double, C.c_double)
+GRAD_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_double, C.c_double, C.c_double,
+                      C.POINTER(C.c_double))
+
+
+class SurfaceDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p", C.c_double * 8), ("phi", PHI_FN), ("grad", GRAD_FN),
+                ("user", C.c_void_p)]
+
+
+class PhysicsDesc(C.Structure):
+    _fields_ = [("beta_kind", C.c_int), ("bp", C.c_double * 4)]
+
+
+class Crossing(C.Structure):
+    _fields_ = [("dir", C.c_int), ("j", C.c_int), ("k", C.c_int), ("id", C.c_int),
+                ("left_loc", C.c_int), ("nw", C.c_int), ("gamma", C.c_double),
+                ("coord", C.c_double * 3), ("normal", C.c_double * 3), ("p", C.c_double * 9),
+                ("pinv_d0", C.c_double), ("beta_in", C.c_double), ("beta_out", C.c_double),
+                ("wl", C.c_double * 9), ("wr", C.c_double * 9)]
+
+
+class SetupInfo(C.Structure):
+    _fields_ = [("n", C.c_int * 3), ("n_cross", C.c_longlong * 3),
+                ("corner_pairs", C.c_int * 3), ("singular_warnings", C.c_int), ("mib", C.c_int)]
+
+
+_lib_handle = None
+
+
+def lib() -> C.CDLL:
+    """The loaded libmts.so (raises if it has not been built)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        L = C.CDLL(LIB_PATH)
+        P, I, D, VP = C.POINTER, C.c_int, C.c_double, C.c_void_p
+        sig = {
+            "mts_last_error": ([], C.c_char_p),
+            "mts_version": ([], C.c_char_p),
+            "mts_setup_create": ([P(GridDesc), P(SurfaceDesc), P(PhysicsDesc), I, I, P(VP)], I),
+            "mts_setup_destroy": ([VP], None),
+            "mts_setup_get_info": ([VP, P(SetupInfo)], I),
+            "mts_setup_indicator": ([VP, VP, C.c_size_t], I),
+            "mts_setup_coords": ([VP, I, VP, C.c_size_t], I),
+            "mts_setup_crossings": ([VP, I, VP, C.c_size_t], I),
+            "mts_ctx_create": ([I, P(VP)], I),
+            "mts_ctx_destroy": ([VP], None),
+            "mts_ctx_device_info": ([VP, C.c_char_p, C.c_size_t, P(C.c_size_t)], I),
+            "mts_instance_create": ([VP, VP, P(PhysicsDesc), I, D, P(VP)], I),
+            "mts_instance_destroy": ([VP], None),
+            "mts_field_upload": ([VP, VP, I], I),
+            "mts_field_download": ([VP, VP, I], I),
+            "mts_initialize": ([VP, D], I),
+            "mts_step": ([VP, D], I),
+            "mts_run": ([VP, D, D, P(I), P(D)], I),
+            "mts_sync": ([VP], I),
+            "mts_error_norms": ([VP, D, P(D), P(D), P(I)], I),
+            "mts_field_device": ([VP, P(VP), P(I), P(VP)], I),
+            "mts_tdma_batched": ([VP, I, I, VP, VP, VP, VP, VP], I),
+        }
+        for name, (args, res) in sig.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib_handle = L
+    return _lib_handle
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise MtsError(rc, lib().mts_last_error().decode())
+
+
+def _dptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def surface(kind: str, *params: float) -> SurfaceDesc:
+    """Surface descriptor: ``ellipsoid`` (a, b, c, r), ``torus`` (R, r),
+    ``bumpy`` (R0, A, m, k)."""
+    s = SurfaceDesc()
+    s.kind = {"ellipsoid": SURF_ELLIPSOID, "torus": SURF_TORUS, "bumpy": SURF_BUMPY}[kind]
+    for i, v in enumerate(params):
+        s.p[i] = float(v)
+    return s
+
+
+def physics(beta: str, *params: float) -> PhysicsDesc:
+    """Diffusion coefficient: ``const`` (beta-, beta+) or ``sincos`` (c, beta+)."""
+    p = PhysicsDesc()
+    p.beta_kind = {"const": BETA_CONST, "sincos": BETA_SINCOS}[beta]
+    for i, v in enumerate(params):
+        p.bp[i] = float(v)
+    return p
+
+
+def grid(n, lo=-1.0, hi=1.0) -> GridDesc:
+    """Grid of n[a] nodes per axis on [lo, hi]^3 (n may be an int)."""
+    g = GridDesc()
+    ns = (n, n, n) if isinstance(n, int) else tuple(n)
+    los = (lo,) * 3 if np.isscalar(lo) else tuple(lo)
+    his = (hi,) * 3 if np.isscalar(hi) else tuple(hi)
+    for a in range(3):
+        g.n[a] = int(ns[a])
+        g.lo[a] = float(los[a])
+        g.hi[a] = float(his[a])
+    return g
+
+
+CROSSING_DTYPE = np.dtype([
+    ("dir", "i4"), ("j", "i4"), ("k", "i4"), ("id", "i4"), ("left_loc", "i4"), ("nw", "i4"),
+    ("gamma", "f8"), ("coord", "f8", 3), ("normal", "f8", 3), ("p", "f8", 9),
+    ("pinv_d0", "f8"), ("beta_in", "f8"), ("beta_out", "f8"), ("wl", "f8", 9),
+    ("wr", "f8", 9)], align=True)
+assert CROSSING_DTYPE.itemsize == C.sizeof(Crossing)
+
+
+class Setup:
+    """Mesh + Intersections: host FP64 setup of one geometry."""
+
+    def __init__(self, g: GridDesc, surf: SurfaceDesc, phys: PhysicsDesc, mib: int = 1,
+                 accuracy: int = 2):
+        self._h = C.c_void_p()
+        self.grid, self.surface, self.physics = g, surf, phys
+        _check(lib().mts_setup_create(C.byref(g), C.byref(surf), C.byref(phys), mib, accuracy,
+                                      C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().mts_setup_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def info(self) -> SetupInfo:
+        i = SetupInfo()
+        _check(lib().mts_setup_get_info(self._h, C.byref(i)))
+        return i
+
+    @property
+    def shape(self):
+        i = self.info()
+        return (i.n[2], i.n[1], i.n[0])  # (nz, ny, nx) of the x-fastest arrays
+
+    def indicator(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.int8)
+        _check(lib().mts_setup_indicator(self._h, _dptr(out), out.size))
+        return out
+
+    def coords(self, axis: int) -> np.ndarray:
+        n = self.info().n[axis]
+        out = np.empty(n, dtype=np.float64)
+        _check(lib().mts_setup_coords(self._h, axis, _dptr(out), n))
+        return out
+
+    def crossings(self, d: int) -> np.ndarray:
+        n = int(self.info().n_cross[d])
+        out = np.zeros(max(n, 1), dtype=CROSSING_DTYPE)
+        _check(lib().mts_setup_crossings(self._h, d, _dptr(out), n))
+        return out[:n]
+
+
+class Context:
+    """A GPU (one per process here)."""
+
+    def __init__(self, device: int = 0):
+        self._h = C.c_void_p()
+        _check(lib().mts_ctx_create(device, C.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().mts_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        hbm = C.c_size_t()
+        _check(lib().mts_ctx_device_info(self._h, name, 256, C.byref(hbm)))
+        return name.value.decode(), hbm.value
+
+    def tdma_batched(self, a, b, c, r) -> np.ndarray:
+        """Solve nlines tridiagonal systems (arrays shaped (nlines, n))."""
+        a, b, c, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c, r))
+        nlines, n = a.shape
+        u = np.empty_like(a)
+        _check(lib().mts_tdma_batched(self._h, n, nlines, _dptr(a), _dptr(b), _dptr(c),
+                                      _dptr(r), _dptr(u)))
+        return u
+
+
+class Instance:
+    """Douglas_ADI / LOD / TS solver resident on the GPU."""
+
+    def __init__(self, ctx: Context, setup: Setup, method, dt: float):
+        self._h = C.c_void_p()
+        self.ctx, self.setup = ctx, setup
+        self.method = METHODS[method] if isinstance(method, str) else int(method)
+        self.dt = float(dt)
+        _check(lib().mts_instance_create(ctx.handle, setup.handle, C.byref(setup.physics),
+                                         self.method, self.dt, C.byref(self._h)))
+        self.shape = setup.shape
+
+    def close(self):
+        if self._h:
+            lib().mts_instance_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, u: np.ndarray, layout: int = LAYOUT_XFAST):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        _check(lib().mts_field_upload(self._h, _dptr(u), layout))
+
+    def download(self, layout: int = LAYOUT_XFAST) -> np.ndarray:
+        nz, ny, nx = self.shape
+        out = np.empty((nz, ny, nx) if layout == LAYOUT_XFAST else (nx, ny, nz))
+        _check(lib().mts_field_download(self._h, _dptr(out), layout))
+        return out
+
+    def initialize(self, t0: float = 0.0):
+        _check(lib().mts_initialize(self._h, t0))
+
+    def step(self, t_next: float):
+        _check(lib().mts_step(self._h, t_next))
+
+    def run(self, t0: float, t_finish: float):
+        steps, t_end = C.c_int(), C.c_double()
+        _check(lib().mts_run(self._h, t0, t_finish, C.byref(steps), C.byref(t_end)))
+        return steps.value, t_end.value
+
+    def sync(self):
+        _check(lib().mts_sync(self._h))
+
+    def error(self, t: float):
+        """(L-inf, L2, argmax (ix, iy, iz)) of u - exact at time t."""
+        linf, l2 = C.c_double(), C.c_double()
+        am = (C.c_int * 3)()
+        _check(lib().mts_error_norms(self._h, t, C.byref(linf), C.byref(l2), am))
+        return linf.value, l2.value, (am[0], am[1], am[2])
+
+    def device_field(self):
+        p, pitch, st = C.c_void_p(), C.c_int(), C.c_void_p()
+        _check(lib().mts_field_device(self._h, C.byref(p), C.byref(pitch), C.byref(st)))
+        return p.value, pitch.value, st.value
+
+
+def version() -> str:
+    return lib().mts_version().decode()

@@ -1,0 +1,577 @@
+// C ABI of the Matched-TS per-step path (include/mts.h).
+#include "mts.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "host_setup.hpp"
+#include "host_table.hpp"
+#include "kernels.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define MTS_HIP(call)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (call);                                                                \
+    if (e_ != hipSuccess)                                                                  \
+      return fail(MTS_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+  } while (0)
+
+std::unique_ptr<mts::Surface> make_surface(const mts_surface_desc *sd, std::string &err) {
+  switch (sd->kind) {
+    case MTS_SURF_ELLIPSOID:
+      return std::unique_ptr<mts::Surface>(
+          new mts::SurfaceEllipsoid(sd->p[0], sd->p[1], sd->p[2], sd->p[3]));
+    case MTS_SURF_TORUS:
+      return std::unique_ptr<mts::Surface>(new mts::SurfaceTorus(sd->p[0], sd->p[1]));
+    case MTS_SURF_BUMPY:
+      return std::unique_ptr<mts::Surface>(
+          new mts::SurfaceBumpy(sd->p[0], sd->p[1], sd->p[2], sd->p[3]));
+    case MTS_SURF_CALLBACK:
+      if (!sd->phi || !sd->grad) {
+        err = "callback surface needs phi and grad";
+        return nullptr;
+      }
+      return std::unique_ptr<mts::Surface>(new mts::SurfaceCallback(sd->phi, sd->grad, sd->user));
+    default:
+      err = "unknown surface kind";
+      return nullptr;
+  }
+}
+
+bool make_beta(const mts_physics_desc *pd, mts::BetaModel &b, std::string &err) {
+  if (pd->beta_kind != MTS_BETA_CONST && pd->beta_kind != MTS_BETA_SINCOS) {
+    err = "unknown beta kind";
+    return false;
+  }
+  b.kind = pd->beta_kind;
+  for (int i = 0; i < 4; i++) b.p[i] = pd->bp[i];
+  return true;
+}
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t count) {
+    n = count;
+    if (count == 0) return hipSuccess;
+    return hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
+  }
+  hipError_t upload(const T *h, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  }
+};
+
+}  // namespace
+
+struct mts_setup {
+  mts::SetupResult res;
+  mts::BetaModel beta;
+};
+
+struct mts_ctx {
+  int device = 0;
+  hipDeviceProp_t prop;
+};
+
+struct mts_instance {
+  mts_ctx *ctx = nullptr;
+  int method = 0;
+  double dt = 0.0, theta_dt = 0.0;
+  mts::Grid grid;
+  int px = 0;
+  size_t field_elems = 0;
+  hipStream_t stream = nullptr;
+  DevBuf<double> u, r;            // solution; Douglas stage scratch
+  DevBuf<int8_t> ind;
+  DevBuf<double> tab[3];
+  DevBuf<int> blk_ptr[3];
+  DevBuf<mts::BlockHdr> blk_hdr[3];
+  DevBuf<double> blk_val[3];
+  DevBuf<double> cr_val[3];
+  DevBuf<int> spec;
+  int nspec = 0;
+  DevBuf<double> part_max, part_sq;
+  DevBuf<long long> part_idx;
+  mts::PhysConst phys;
+  mts::DevGeom geom;
+  mts::DevMib mib[3];
+  ~mts_instance() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+extern "C" {
+
+const char *mts_last_error(void) { return g_err.c_str(); }
+const char *mts_version(void) { return "mts-amd 0.1 (gfx950)"; }
+
+int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
+                     const mts_physics_desc *physics, int mib, int accuracy, mts_setup **out) {
+  if (!grid || !surface || !physics || !out) return fail(MTS_E_ARG, "null argument");
+  *out = nullptr;
+  for (int a = 0; a < 3; a++) {
+    if (grid->n[a] < 5) return fail(MTS_E_ARG, "each axis needs at least 5 nodes");
+    if (!(grid->hi[a] > grid->lo[a])) return fail(MTS_E_ARG, "empty domain");
+  }
+  std::string err;
+  std::unique_ptr<mts::Surface> surf = make_surface(surface, err);
+  if (!surf) return fail(MTS_E_ARG, err);
+  std::unique_ptr<mts_setup> s(new (std::nothrow) mts_setup());
+  if (!s) return fail(MTS_E_NOMEM, "out of host memory");
+  if (!make_beta(physics, s->beta, err)) return fail(MTS_E_ARG, err);
+  mts::Grid g = mts::make_grid(grid->lo, grid->hi, grid->n);
+  mts::BuildOptions opt;
+  opt.mib = mib;
+  opt.accuracy = accuracy;
+  try {
+    err = mts::build_setup(g, *surf, s->beta, opt, s->res);
+  } catch (const std::bad_alloc &) {
+    return fail(MTS_E_NOMEM, "out of host memory in setup");
+  }
+  if (!err.empty()) return fail(MTS_E_GEOMETRY, err);
+  *out = s.release();
+  return MTS_OK;
+}
+
+void mts_setup_destroy(mts_setup *s) { delete s; }
+
+int mts_setup_get_info(const mts_setup *s, mts_setup_info *info) {
+  if (!s || !info) return fail(MTS_E_ARG, "null argument");
+  for (int a = 0; a < 3; a++) {
+    info->n[a] = s->res.grid.n[a];
+    info->n_cross[a] = (long long)s->res.cross[a].size();
+    info->corner_pairs[a] = s->res.corner_pairs[a];
+  }
+  info->singular_warnings = s->res.singular_warnings;
+  info->mib = s->res.mib;
+  return MTS_OK;
+}
+
+int mts_setup_indicator(const mts_setup *s, int8_t *out, size_t count) {
+  if (!s || !out) return fail(MTS_E_ARG, "null argument");
+  if (count < s->res.ind.size()) return fail(MTS_E_ARG, "indicator buffer too small");
+  std::memcpy(out, s->res.ind.data(), s->res.ind.size());
+  return MTS_OK;
+}
+
+int mts_setup_coords(const mts_setup *s, int axis, double *out, size_t count) {
+  if (!s || !out || axis < 0 || axis > 2) return fail(MTS_E_ARG, "bad argument");
+  const std::vector<double> &c = s->res.grid.c[axis];
+  if (count < c.size()) return fail(MTS_E_ARG, "coordinate buffer too small");
+  std::memcpy(out, c.data(), c.size() * sizeof(double));
+  return MTS_OK;
+}
+
+int mts_setup_crossings(const mts_setup *s, int dir, mts_crossing *out, size_t count) {
+  if (!s || !out || dir < 0 || dir > 2) return fail(MTS_E_ARG, "bad argument");
+  const std::vector<mts::Crossing> &cs = s->res.cross[dir];
+  if (count < cs.size()) return fail(MTS_E_ARG, "crossing buffer too small");
+  for (size_t i = 0; i < cs.size(); i++) {
+    const mts::Crossing &c = cs[i];
+    mts_crossing &o = out[i];
+    std::memset(&o, 0, sizeof(o));
+    o.dir = c.dir;
+    o.j = c.j;
+    o.k = c.k;
+    o.id = c.id;
+    o.left_loc = c.left_loc;
+    o.nw = c.nw;
+    o.gamma = c.gamma;
+    for (int a = 0; a < 3; a++) {
+      o.coord[a] = c.coord[a];
+      o.normal[a] = c.normal[a];
+      for (int b = 0; b < 3; b++) o.p[a * 3 + b] = c.p[a][b];
+    }
+    o.pinv_d0 = c.pinv_d0;
+    o.beta_in = c.beta_in;
+    o.beta_out = c.beta_out;
+    for (int w = 0; w < 9; w++) {
+      o.wl[w] = c.wl[w];
+      o.wr[w] = c.wr[w];
+    }
+  }
+  return MTS_OK;
+}
+
+int mts_ctx_create(int device, mts_ctx **out) {
+  if (!out) return fail(MTS_E_ARG, "null argument");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return fail(MTS_E_DEVICE, "no HIP device available (the per-step path runs on the GPU only)");
+  if (device < 0 || device >= count) return fail(MTS_E_ARG, "device index out of range");
+  std::unique_ptr<mts_ctx> c(new mts_ctx());
+  c->device = device;
+  MTS_HIP(hipSetDevice(device));
+  MTS_HIP(hipGetDeviceProperties(&c->prop, device));
+  *out = c.release();
+  return MTS_OK;
+}
+
+void mts_ctx_destroy(mts_ctx *ctx) { delete ctx; }
+
+int mts_ctx_device_info(mts_ctx *ctx, char *name, size_t name_len, size_t *hbm_bytes) {
+  if (!ctx) return fail(MTS_E_ARG, "null context");
+  if (name && name_len) std::snprintf(name, name_len, "%s (%s)", ctx->prop.name, ctx->prop.gcnArchName);
+  if (hbm_bytes) *hbm_bytes = ctx->prop.totalGlobalMem;
+  return MTS_OK;
+}
+
+int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                        int method, double dt, mts_instance **out) {
+  if (!ctx || !setup || !physics || !out) return fail(MTS_E_ARG, "null argument");
+  *out = nullptr;
+  if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
+  if (!(dt > 0.0)) return fail(MTS_E_ARG, "dt must be positive");
+  mts::BetaModel beta;
+  std::string err;
+  if (!make_beta(physics, beta, err)) return fail(MTS_E_ARG, err);
+  if (beta.kind != setup->beta.kind || beta.p[0] != setup->beta.p[0] ||
+      beta.p[1] != setup->beta.p[1])
+    return fail(MTS_E_ARG, "physics differs from the one the setup was built with");
+  MTS_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<mts_instance> in(new mts_instance());
+  in->ctx = ctx;
+  in->method = method;
+  in->dt = dt;
+  in->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
+  in->grid = setup->res.grid;
+  const mts::Grid &g = in->grid;
+  for (int d = 0; d < 3; d++) {
+    int n = g.n[d];
+    int p = n / 2 > 64 ? 64 : n / 2;
+    if ((n + p - 1) / p > 16) return fail(MTS_E_ARG, "line length above 1024 is not supported");
+    size_t tiles = (size_t)16 * (n + 1);
+    size_t need = tiles * sizeof(double) * 2 + tiles;
+    if (need > ctx->prop.sharedMemPerBlock)
+      return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
+  }
+  in->px = (g.n[0] + 15) / 16 * 16;
+  in->field_elems = (size_t)in->px * g.n[1] * g.n[2];
+  MTS_HIP(hipStreamCreateWithFlags(&in->stream, hipStreamNonBlocking));
+
+  mts::DeviceTableHost t;
+  try {
+    mts::build_device_table(setup->res, beta, in->theta_dt, method == MTS_ADI, t);
+  } catch (const std::bad_alloc &) {
+    return fail(MTS_E_NOMEM, "out of host memory building the MIB table");
+  }
+  in->phys = t.phys;
+
+  MTS_HIP(in->u.alloc(in->field_elems));
+  MTS_HIP(hipMemset(in->u.p, 0, in->field_elems * sizeof(double)));
+  if (method == MTS_ADI) {
+    MTS_HIP(in->r.alloc(in->field_elems));
+    MTS_HIP(hipMemset(in->r.p, 0, in->field_elems * sizeof(double)));
+  }
+  {
+    std::vector<int8_t> ind(in->field_elems, 1);
+    for (int iz = 0; iz < g.n[2]; iz++)
+      for (int iy = 0; iy < g.n[1]; iy++)
+        std::memcpy(&ind[((size_t)iz * g.n[1] + iy) * in->px], &setup->res.ind[g.idx(0, iy, iz)],
+                    g.n[0]);
+    MTS_HIP(in->ind.upload(ind.data(), ind.size()));
+  }
+  for (int a = 0; a < 3; a++) MTS_HIP(in->tab[a].upload(t.tab[a].data(), t.tab[a].size()));
+  for (int d = 0; d < 3; d++) {
+    MTS_HIP(in->blk_ptr[d].upload(t.blk_ptr[d].data(), t.blk_ptr[d].size()));
+    MTS_HIP(in->blk_hdr[d].upload(t.blk_hdr[d].data(), t.blk_hdr[d].size()));
+    MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size()));
+    MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size()));
+  }
+  in->nspec = (int)(t.spec.size() / mts::kSpecLen);
+  MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size()));
+  mts::DevGeom &dg = in->geom;
+  for (int a = 0; a < 3; a++) {
+    dg.n[a] = g.n[a];
+    dg.tab[a] = in->tab[a].p;
+  }
+  dg.px = in->px;
+  dg.ind = in->ind.p;
+  for (int d = 0; d < 3; d++) {
+    in->mib[d].blk_ptr = in->blk_ptr[d].p;
+    in->mib[d].hdr = in->blk_hdr[d].p;
+    in->mib[d].val = in->blk_val[d].p;
+    in->mib[d].cr = in->cr_val[d].p;
+  }
+  int nb = mts::norms_blocks(dg);
+  MTS_HIP(in->part_max.alloc(nb));
+  MTS_HIP(in->part_sq.alloc(nb));
+  MTS_HIP(in->part_idx.alloc(nb));
+  *out = in.release();
+  return MTS_OK;
+}
+
+void mts_instance_destroy(mts_instance *inst) {
+  if (!inst) return;
+  (void)hipSetDevice(inst->ctx->device);
+  (void)hipStreamSynchronize(inst->stream);
+  delete inst;
+}
+
+static void pack_field(const mts_instance *in, const double *host, int layout,
+                       std::vector<double> &buf) {
+  const mts::Grid &g = in->grid;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  buf.assign(in->field_elems, 0.0);
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++) {
+      double *row = &buf[((size_t)iz * ny + iy) * in->px];
+      if (layout == MTS_LAYOUT_XFAST) {
+        std::memcpy(row, host + ((size_t)iz * ny + iy) * nx, nx * sizeof(double));
+      } else {
+        for (int ix = 0; ix < nx; ix++) row[ix] = host[((size_t)ix * ny + iy) * nz + iz];
+      }
+    }
+}
+
+int mts_field_upload(mts_instance *in, const double *host, int layout) {
+  if (!in || !host) return fail(MTS_E_ARG, "null argument");
+  if (layout != MTS_LAYOUT_XFAST && layout != MTS_LAYOUT_REF) return fail(MTS_E_ARG, "bad layout");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  std::vector<double> buf;
+  pack_field(in, host, layout, buf);
+  MTS_HIP(hipMemcpyAsync(in->u.p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
+                         in->stream));
+  MTS_HIP(hipStreamSynchronize(in->stream));
+  return MTS_OK;
+}
+
+int mts_field_download(mts_instance *in, double *host, int layout) {
+  if (!in || !host) return fail(MTS_E_ARG, "null argument");
+  if (layout != MTS_LAYOUT_XFAST && layout != MTS_LAYOUT_REF) return fail(MTS_E_ARG, "bad layout");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  std::vector<double> buf(in->field_elems);
+  MTS_HIP(hipMemcpyAsync(buf.data(), in->u.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
+                         in->stream));
+  MTS_HIP(hipStreamSynchronize(in->stream));
+  const mts::Grid &g = in->grid;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  for (int iz = 0; iz < nz; iz++)
+    for (int iy = 0; iy < ny; iy++) {
+      const double *row = &buf[((size_t)iz * ny + iy) * in->px];
+      if (layout == MTS_LAYOUT_XFAST) {
+        std::memcpy(host + ((size_t)iz * ny + iy) * nx, row, nx * sizeof(double));
+      } else {
+        for (int ix = 0; ix < nx; ix++) host[((size_t)ix * ny + iy) * nz + iz] = row[ix];
+      }
+    }
+  return MTS_OK;
+}
+
+int mts_initialize(mts_instance *in, double t0) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  MTS_HIP(mts::launch_init(in->geom, in->u.p, std::exp(-t0), in->stream));
+  return MTS_OK;
+}
+
+static mts::LineArgs base_args(mts_instance *in) {
+  mts::LineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = in->geom;
+  a.P = in->phys;
+  a.theta_dt = in->theta_dt;
+  a.dt = in->dt;
+  return a;
+}
+
+int mts_step(mts_instance *in, double t_next) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  hipStream_t st = in->stream;
+  const double dt = in->dt;
+  const double e1 = std::exp(-t_next);
+  using namespace mts;
+  if (in->method == MTS_ADI) {
+    Rhs1Args r1;
+    r1.g = in->geom;
+    r1.P = in->phys;
+    r1.u = in->u.p;
+    r1.R = in->r.p;
+    r1.dt = dt;
+    r1.e = e1;
+    MTS_HIP(launch_rhs1(r1, st));
+    SpecArgs sp;
+    sp.g = in->geom;
+    sp.P = in->phys;
+    for (int d = 0; d < 3; d++) sp.mib[d] = in->mib[d];
+    sp.spec = in->spec.p;
+    sp.nspec = in->nspec;
+    sp.u = in->u.p;
+    sp.R = in->r.p;
+    sp.dt = dt;
+    sp.e = e1;
+    MTS_HIP(launch_spec(sp, st));
+    LineArgs a = base_args(in);
+    a.e_out = e1;
+    a.e_jump = e1;
+    a.mib = in->mib[0];
+    a.A = in->r.p;
+    a.out = in->r.p;
+    MTS_HIP(launch_line(0, kRhsCopy | kFlagSolve, a, st));
+    a.mib = in->mib[1];
+    a.S = in->u.p;
+    MTS_HIP(launch_line(1, kRhsDouglas | kFlagSolve, a, st));
+    a.mib = in->mib[2];
+    a.out = in->u.p;
+    MTS_HIP(launch_line(2, kRhsDouglas | kFlagSolve, a, st));
+    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st));
+  } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
+    const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;
+    LineArgs a = base_args(in);
+    a.e_out = e1;
+    a.e_jump = e1;
+    a.A = in->u.p;
+    a.out = in->u.p;
+    a.mib = in->mib[0];
+    MTS_HIP(launch_line(0, rhs | kFlagSolve | kFlagLhsJump, a, st));
+    a.in_analytic = 1;
+    a.e_in = e1;
+    a.mib = in->mib[1];
+    MTS_HIP(launch_line(1, rhs | kFlagSolve | kFlagLhsJump, a, st));
+    a.mib = in->mib[2];
+    a.e_f1 = e1;
+    a.c_f1 = dt;
+    MTS_HIP(launch_line(2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st));
+    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, dt, st));
+  } else {
+    const double en = std::exp(-(t_next - dt));
+    const double eh = std::exp(-(t_next - 0.5 * dt));
+    LineArgs a = base_args(in);
+    a.e_jump = e1;
+    a.A = in->u.p;
+    a.out = in->u.p;
+    // explicit (I + dt/2 L_x), (I + dt/2 L_y) with boundary values at t^{n+1/2}
+    a.e_out = eh;
+    a.mib = in->mib[0];
+    MTS_HIP(launch_line(0, kRhsCN, a, st));
+    a.in_analytic = 1;
+    a.e_in = eh;
+    a.mib = in->mib[1];
+    MTS_HIP(launch_line(1, kRhsCN, a, st));
+    // explicit z, sources at t^n and t^{n+1}, implicit z
+    a.mib = in->mib[2];
+    a.e_out = e1;
+    a.e_f1 = en;
+    a.c_f1 = 0.5 * dt;
+    a.e_f2 = e1;
+    a.c_f2 = 0.5 * dt;
+    MTS_HIP(launch_line(2, kRhsTSZ | kFlagSolve | kFlagLhsJump, a, st));
+    // implicit y, x
+    a.e_in = e1;
+    a.mib = in->mib[1];
+    MTS_HIP(launch_line(1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st));
+    a.mib = in->mib[0];
+    MTS_HIP(launch_line(0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st));
+    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st));
+  }
+  return MTS_OK;
+}
+
+int mts_run(mts_instance *in, double t0, double t_finish, int *steps, double *t_end) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  int loop = (int)(t_finish / in->dt);
+  double t = t0;
+  for (int j = 0; j < loop; j++) {
+    t += in->dt;
+    int rc = mts_step(in, t);
+    if (rc) return rc;
+  }
+  if (steps) *steps = loop;
+  if (t_end) *t_end = t;
+  return MTS_OK;
+}
+
+int mts_sync(mts_instance *in) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  MTS_HIP(hipStreamSynchronize(in->stream));
+  return MTS_OK;
+}
+
+int mts_error_norms(mts_instance *in, double t, double *linf, double *l2, int argmax[3]) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  int nparts = 0;
+  MTS_HIP(mts::launch_norms(in->geom, in->u.p, std::exp(-t), in->part_max.p, in->part_idx.p,
+                            in->part_sq.p, &nparts, in->stream));
+  std::vector<double> pm(nparts), ps(nparts);
+  std::vector<long long> pi(nparts);
+  MTS_HIP(hipMemcpyAsync(pm.data(), in->part_max.p, nparts * sizeof(double),
+                         hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipMemcpyAsync(ps.data(), in->part_sq.p, nparts * sizeof(double),
+                         hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipMemcpyAsync(pi.data(), in->part_idx.p, nparts * sizeof(long long),
+                         hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipStreamSynchronize(in->stream));
+  double mx = -1.0, sq = 0.0;
+  long long mi = 0;
+  for (int i = 0; i < nparts; i++) {
+    if (pm[i] > mx || (pm[i] == mx && pi[i] < mi)) {
+      mx = pm[i];
+      mi = pi[i];
+    }
+    sq += ps[i];
+  }
+  const mts::Grid &g = in->grid;
+  const long long ny = g.n[1], nz = g.n[2];
+  if (linf) *linf = mx;
+  if (l2) *l2 = std::sqrt(sq / (1.0 * g.n[0] * g.n[1] * g.n[2]));
+  if (argmax) {
+    argmax[0] = (int)(mi / (ny * nz));
+    argmax[1] = (int)((mi / nz) % ny);
+    argmax[2] = (int)(mi % nz);
+  }
+  return MTS_OK;
+}
+
+int mts_field_device(mts_instance *in, double **dptr, int *pitch, void **stream) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  if (dptr) *dptr = in->u.p;
+  if (pitch) *pitch = in->px;
+  if (stream) *stream = in->stream;
+  return MTS_OK;
+}
+
+int mts_tdma_batched(mts_ctx *ctx, int n, int nlines, const double *a, const double *b,
+                     const double *c, const double *r, double *u) {
+  if (!ctx || !a || !b || !c || !r || !u) return fail(MTS_E_ARG, "null argument");
+  if (n < 2 || n > 1024 || nlines < 1) return fail(MTS_E_ARG, "bad sizes");
+  MTS_HIP(hipSetDevice(ctx->device));
+  size_t cnt = (size_t)n * nlines;
+  DevBuf<double> da, db, dc, dr, du;
+  MTS_HIP(da.upload(a, cnt));
+  MTS_HIP(db.upload(b, cnt));
+  MTS_HIP(dc.upload(c, cnt));
+  MTS_HIP(dr.upload(r, cnt));
+  MTS_HIP(du.alloc(cnt));
+  MTS_HIP(mts::launch_tdma(n, nlines, da.p, db.p, dc.p, dr.p, du.p, nullptr));
+  MTS_HIP(hipDeviceSynchronize());
+  MTS_HIP(hipMemcpy(u, du.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
+  return MTS_OK;
+}
+
+}  // extern "C"
