@@ -153,6 +153,19 @@ int mts_error_norms(mts_instance *inst, double t, double *linf, double *l2, int 
  * instance's HIP stream, for callers that keep data on the device. */
 int mts_field_device(mts_instance *inst, double **dptr, int *pitch, void **stream);
 
+/* Per-stage timing with HIP events on the instance's stream (the reference
+ * only times the whole run, main.cc:119-124).  When enabled, every launch of
+ * mts_step is bracketed by events; mts_profile_read synchronizes and returns
+ * per-stage launch counts and total milliseconds. */
+typedef struct {
+  char name[32];
+  int launches;
+  double total_ms;
+} mts_stage_time;
+int mts_profile_enable(mts_instance *inst, int on);
+int mts_profile_reset(mts_instance *inst);
+int mts_profile_read(mts_instance *inst, mts_stage_time *out, int capacity, int *count);
+
 /* Batched tridiagonal solve (the reference's TDMA, douglas_adi.cc:1784-1807,
  * for many lines at once): nlines systems of length n, host arrays line-major;
  * a[l*n] and c[l*n + n - 1] are ignored. */

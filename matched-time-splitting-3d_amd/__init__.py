@@ -68,6 +68,10 @@ class Crossing(C.Structure):
                 ("wl", C.c_double * 9), ("wr", C.c_double * 9)]
 
 
+class StageTime(C.Structure):
+    _fields_ = [("name", C.c_char * 32), ("launches", C.c_int), ("total_ms", C.c_double)]
+
+
 class SetupInfo(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("n_cross", C.c_longlong * 3),
                 ("corner_pairs", C.c_int * 3), ("singular_warnings", C.c_int), ("mib", C.c_int)]
@@ -108,6 +112,9 @@ def lib() -> C.CDLL:
             "mts_error_norms": ([VP, D, P(D), P(D), P(I)], I),
             "mts_field_device": ([VP, P(VP), P(I), P(VP)], I),
             "mts_tdma_batched": ([VP, I, I, VP, VP, VP, VP, VP], I),
+            "mts_profile_enable": ([VP, I], I),
+            "mts_profile_reset": ([VP], I),
+            "mts_profile_read": ([VP, VP, I, P(I)], I),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -310,6 +317,21 @@ class Instance:
         am = (C.c_int * 3)()
         _check(lib().mts_error_norms(self._h, t, C.byref(linf), C.byref(l2), am))
         return linf.value, l2.value, (am[0], am[1], am[2])
+
+    def profile(self, on: bool = True):
+        """Bracket every stage launch with HIP events (per-stage timing)."""
+        _check(lib().mts_profile_enable(self._h, 1 if on else 0))
+
+    def profile_reset(self):
+        _check(lib().mts_profile_reset(self._h))
+
+    def profile_read(self):
+        """{stage: (launches, total_ms)} of the timed launches so far."""
+        buf = (StageTime * 64)()
+        cnt = C.c_int()
+        _check(lib().mts_profile_read(self._h, buf, 64, C.byref(cnt)))
+        return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms)
+                for i in range(min(cnt.value, 64))}
 
     def device_field(self):
         p, pitch, st = C.c_void_p(), C.c_int(), C.c_void_p()

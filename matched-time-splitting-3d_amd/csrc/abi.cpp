@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <new>
 #include <string>
@@ -116,10 +117,77 @@ struct mts_instance {
   mts::PhysConst phys;
   mts::DevGeom geom;
   mts::DevMib mib[3];
+  // per-stage event timing
+  bool profile = false;
+  std::vector<hipEvent_t> ev_pool;
+  size_t ev_used = 0;
+  std::vector<std::pair<int, size_t>> pending;  // stage id, first event index
+  std::vector<std::string> stage_names;
+  std::map<std::string, int> stage_id;
+  std::vector<double> stage_ms;
+  std::vector<int> stage_n;
   ~mts_instance() {
     if (stream) (void)hipStreamDestroy(stream);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
+  int stage(const char *name) {
+    auto it = stage_id.find(name);
+    if (it != stage_id.end()) return it->second;
+    int id = (int)stage_names.size();
+    stage_id[name] = id;
+    stage_names.push_back(name);
+    stage_ms.push_back(0.0);
+    stage_n.push_back(0);
+    return id;
+  }
+  hipEvent_t event() {
+    if (ev_used == ev_pool.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return nullptr;
+      ev_pool.push_back(e);
+    }
+    return ev_pool[ev_used++];
+  }
+  hipError_t drain() {
+    if (pending.empty()) return hipSuccess;
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return e;
+    for (auto &p : pending) {
+      float ms = 0.f;
+      e = hipEventElapsedTime(&ms, ev_pool[p.second], ev_pool[p.second + 1]);
+      if (e != hipSuccess) return e;
+      stage_ms[p.first] += ms;
+      stage_n[p.first] += 1;
+    }
+    pending.clear();
+    ev_used = 0;
+    return hipSuccess;
   }
 };
+
+namespace {
+// Runs one launch, bracketed by events when profiling is on.
+template <class F>
+hipError_t timed(mts_instance *in, const char *name, F launch) {
+  if (!in->profile) return launch();
+  if (in->ev_used + 2 > 4096) {
+    hipError_t e = in->drain();
+    if (e != hipSuccess) return e;
+  }
+  int id = in->stage(name);
+  size_t first = in->ev_used;
+  hipEvent_t a = in->event(), b = in->event();
+  if (!a || !b) return hipErrorOutOfMemory;
+  hipError_t e = hipEventRecord(a, in->stream);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, in->stream);
+  if (e != hipSuccess) return e;
+  in->pending.push_back({id, first});
+  return hipSuccess;
+}
+}  // namespace
 
 extern "C" {
 
@@ -413,7 +481,7 @@ int mts_step(mts_instance *in, double t_next) {
     r1.R = in->r.p;
     r1.dt = dt;
     r1.e = e1;
-    MTS_HIP(launch_rhs1(r1, st));
+    MTS_HIP(timed(in, "adi_rhs1", [&] { return launch_rhs1(r1, st); }));
     SpecArgs sp;
     sp.g = in->geom;
     sp.P = in->phys;
@@ -424,21 +492,23 @@ int mts_step(mts_instance *in, double t_next) {
     sp.R = in->r.p;
     sp.dt = dt;
     sp.e = e1;
-    MTS_HIP(launch_spec(sp, st));
+    MTS_HIP(timed(in, "adi_spec", [&] { return launch_spec(sp, st); }));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
     a.mib = in->mib[0];
     a.A = in->r.p;
     a.out = in->r.p;
-    MTS_HIP(launch_line(0, kRhsCopy | kFlagSolve, a, st));
+    MTS_HIP(timed(in, "adi_x", [&] { return launch_line(0, kRhsCopy | kFlagSolve, a, st); }));
     a.mib = in->mib[1];
     a.S = in->u.p;
-    MTS_HIP(launch_line(1, kRhsDouglas | kFlagSolve, a, st));
+    MTS_HIP(timed(in, "adi_y", [&] { return launch_line(1, kRhsDouglas | kFlagSolve, a, st); }));
     a.mib = in->mib[2];
     a.out = in->u.p;
-    MTS_HIP(launch_line(2, kRhsDouglas | kFlagSolve, a, st));
-    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st));
+    MTS_HIP(timed(in, "adi_z", [&] { return launch_line(2, kRhsDouglas | kFlagSolve, a, st); }));
+    MTS_HIP(timed(in, "faces", [&] {
+      return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
+    }));
   } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
     const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;
     LineArgs a = base_args(in);
@@ -447,16 +517,20 @@ int mts_step(mts_instance *in, double t_next) {
     a.A = in->u.p;
     a.out = in->u.p;
     a.mib = in->mib[0];
-    MTS_HIP(launch_line(0, rhs | kFlagSolve | kFlagLhsJump, a, st));
+    MTS_HIP(timed(in, "lod_x", [&] { return launch_line(0, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
     a.in_analytic = 1;
     a.e_in = e1;
     a.mib = in->mib[1];
-    MTS_HIP(launch_line(1, rhs | kFlagSolve | kFlagLhsJump, a, st));
+    MTS_HIP(timed(in, "lod_y", [&] { return launch_line(1, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[2];
     a.e_f1 = e1;
     a.c_f1 = dt;
-    MTS_HIP(launch_line(2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st));
-    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, dt, st));
+    MTS_HIP(timed(in, "lod_z", [&] {
+      return launch_line(2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
+    }));
+    MTS_HIP(timed(in, "faces", [&] {
+      return launch_faces(in->geom, in->phys, in->u.p, e1, dt, st);
+    }));
   } else {
     const double en = std::exp(-(t_next - dt));
     const double eh = std::exp(-(t_next - 0.5 * dt));
@@ -467,11 +541,11 @@ int mts_step(mts_instance *in, double t_next) {
     // explicit (I + dt/2 L_x), (I + dt/2 L_y) with boundary values at t^{n+1/2}
     a.e_out = eh;
     a.mib = in->mib[0];
-    MTS_HIP(launch_line(0, kRhsCN, a, st));
+    MTS_HIP(timed(in, "ts_ex", [&] { return launch_line(0, kRhsCN, a, st); }));
     a.in_analytic = 1;
     a.e_in = eh;
     a.mib = in->mib[1];
-    MTS_HIP(launch_line(1, kRhsCN, a, st));
+    MTS_HIP(timed(in, "ts_ey", [&] { return launch_line(1, kRhsCN, a, st); }));
     // explicit z, sources at t^n and t^{n+1}, implicit z
     a.mib = in->mib[2];
     a.e_out = e1;
@@ -479,14 +553,16 @@ int mts_step(mts_instance *in, double t_next) {
     a.c_f1 = 0.5 * dt;
     a.e_f2 = e1;
     a.c_f2 = 0.5 * dt;
-    MTS_HIP(launch_line(2, kRhsTSZ | kFlagSolve | kFlagLhsJump, a, st));
+    MTS_HIP(timed(in, "ts_z", [&] { return launch_line(2, kRhsTSZ | kFlagSolve | kFlagLhsJump, a, st); }));
     // implicit y, x
     a.e_in = e1;
     a.mib = in->mib[1];
-    MTS_HIP(launch_line(1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st));
+    MTS_HIP(timed(in, "ts_iy", [&] { return launch_line(1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[0];
-    MTS_HIP(launch_line(0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st));
-    MTS_HIP(launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st));
+    MTS_HIP(timed(in, "ts_ix", [&] { return launch_line(0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "faces", [&] {
+      return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
+    }));
   }
   return MTS_OK;
 }
@@ -553,6 +629,39 @@ int mts_field_device(mts_instance *in, double **dptr, int *pitch, void **stream)
   if (dptr) *dptr = in->u.p;
   if (pitch) *pitch = in->px;
   if (stream) *stream = in->stream;
+  return MTS_OK;
+}
+
+int mts_profile_enable(mts_instance *in, int on) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  MTS_HIP(in->drain());
+  in->profile = on != 0;
+  return MTS_OK;
+}
+
+int mts_profile_reset(mts_instance *in) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  MTS_HIP(in->drain());
+  for (size_t i = 0; i < in->stage_ms.size(); i++) {
+    in->stage_ms[i] = 0.0;
+    in->stage_n[i] = 0;
+  }
+  return MTS_OK;
+}
+
+int mts_profile_read(mts_instance *in, mts_stage_time *out, int capacity, int *count) {
+  if (!in || !count) return fail(MTS_E_ARG, "null argument");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  MTS_HIP(in->drain());
+  int n = (int)in->stage_names.size();
+  *count = n;
+  for (int i = 0; i < n && i < capacity && out; i++) {
+    std::snprintf(out[i].name, sizeof(out[i].name), "%s", in->stage_names[i].c_str());
+    out[i].launches = in->stage_n[i];
+    out[i].total_ms = in->stage_ms[i];
+  }
   return MTS_OK;
 }
 

@@ -1,0 +1,100 @@
+"""Shared fixtures: the product binding, the oracle binding, golden vectors."""
+import glob
+import importlib.util
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PKG_DIR = os.path.join(ROOT, "matched-time-splitting-3d_amd")
+GOLDEN_DIR = os.path.join(ROOT, "tests", "golden")
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: runs the HIP path on an MI355X")
+
+
+def load_mts():
+    mod = sys.modules.get("mts_amd")
+    if mod is None:
+        spec = importlib.util.spec_from_file_location(
+            "mts_amd", os.path.join(PKG_DIR, "__init__.py"),
+            submodule_search_locations=[PKG_DIR])
+        mod = importlib.util.module_from_spec(spec)
+        sys.modules["mts_amd"] = mod
+        spec.loader.exec_module(mod)
+    return mod
+
+
+@pytest.fixture(scope="session")
+def mts():
+    return load_mts()
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import mts_oracle
+    return mts_oracle
+
+
+@pytest.fixture(scope="session")
+def ctx(mts):
+    # The per-step path runs on the GPU only: no device is an error, not a skip.
+    return mts.Context(0)
+
+
+def rel_err(a, b):
+    """Normwise relative difference max|a - b| / max|b|."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.max(np.abs(a - b)) / max(float(np.max(np.abs(b))), 1e-300))
+
+
+GOLDEN_NAMES = sorted(os.path.splitext(os.path.basename(p))[0]
+                      for p in glob.glob(os.path.join(GOLDEN_DIR, "*.npz")))
+
+
+def load_golden(name):
+    z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
+    g = {k: z[k] for k in z.files}
+    g["meta"] = json.loads(str(g["meta"]))
+    return g
+
+
+def golden_crossings(g, d):
+    """Golden crossings of direction d as a structured array (oracle field names)."""
+    ii, dd = g[f"cr{d}_int"], g[f"cr{d}_dbl"]
+    dt = np.dtype([("j", "i4"), ("k", "i4"), ("id", "i4"), ("left_loc", "i4"), ("nw", "i4"),
+                   ("gamma", "f8"), ("coord", "f8", 3), ("normal", "f8", 3),
+                   ("beta_in", "f8"), ("beta_out", "f8"), ("p", "f8", 9), ("wl", "f8", 9),
+                   ("wr", "f8", 9)])
+    out = np.zeros(len(ii), dtype=dt)
+    if len(ii):
+        out["j"], out["k"], out["id"], out["left_loc"], out["nw"] = ii.T
+        out["gamma"] = dd[:, 0]
+        out["coord"] = dd[:, 1:4]
+        out["normal"] = dd[:, 4:7]
+        out["beta_in"] = dd[:, 7]
+        out["beta_out"] = dd[:, 8]
+        out["p"] = dd[:, 9:18]
+        out["wl"] = dd[:, 18:27]
+        out["wr"] = dd[:, 27:36]
+    return out
+
+
+def case_setup(mts, meta):
+    """Product Setup for a golden case's geometry."""
+    n = meta["n"] + 1
+    s = meta["surface"]
+    b = meta["beta"]
+    phys = mts.physics(b[0], *b[1:])
+    st = mts.Setup(mts.grid(n), mts.surface(s[0], *s[1:]), phys, mib=meta["mib"])
+    return st, phys
+
+
+def beta_kind(meta):
+    return {"const": 0, "sincos": 1}[meta["beta"][0]]

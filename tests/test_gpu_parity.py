@@ -1,0 +1,122 @@
+"""GPU parity: the HIP path against the reference's golden fields and the oracle.
+
+Tolerance: 1e-12 normwise relative (north-star bar for the plain FP64 form),
+scaled by the solution's growth for runs the reference's scheme amplifies
+(TS and Douglas at dt = h with strong contrast grow by up to 1e7 over a few
+steps and amplify every rounding difference by as much).  The partitioned
+line solve rounds differently from the reference's sequential Thomas
+algorithm; everything else follows the reference's operation order.
+"""
+import numpy as np
+import pytest
+
+from conftest import (GOLDEN_NAMES, beta_kind, case_setup, golden_crossings, load_golden,
+                      rel_err)
+
+pytestmark = pytest.mark.gpu
+
+
+def growth_tol(u):
+    return 1e-12 * max(1.0, float(np.max(np.abs(u))))
+
+
+@pytest.mark.parametrize("name", GOLDEN_NAMES)
+def test_gpu_matches_reference(mts, ctx, name):
+    g = load_golden(name)
+    meta = g["meta"]
+    print(f"case {name}", flush=True)
+    st, _ = case_setup(mts, meta)
+    inst = mts.Instance(ctx, st, meta["method"], meta["dt"])
+    inst.initialize(0.0)
+    inst.step(0.0 + meta["dt"])
+    u1 = inst.download()
+    assert rel_err(u1, g["u_step1"]) <= growth_tol(g["u_step1"])
+    inst.initialize(0.0)
+    steps, t_end = inst.run(0.0, meta["T"])
+    assert steps == int(g["summary"][0])
+    assert t_end == g["summary"][1]
+    uf = inst.download()
+    assert rel_err(uf, g["u_final"]) <= growth_tol(g["u_final"])
+    linf, l2, am = inst.error(t_end)
+    tol = growth_tol(g["u_final"])
+    assert abs(linf - g["summary"][2]) <= tol * max(1.0, g["summary"][2])
+    assert abs(l2 - g["summary"][3]) <= tol * max(1.0, g["summary"][3])
+    inst.close()
+
+
+CASES = [
+    # (surface, args, beta, n intervals, dt, steps)
+    ("ellipsoid", (0.7, 0.5, 0.4, 1 - 2 ** -20), ("sincos", 10, 1), 64, 1 / 32, 3),
+    ("torus", (0.5, 0.2), ("const", 0.01, 1), 48, 1 / 24, 3),
+    ("bumpy", (0.5 * (1 - 2 ** -20), 0.1, 5, 3), ("const", 1000, 1), 40, 1 / 20, 2),
+    ("ellipsoid", (0.8, 0.55, 0.45, 0.95), ("const", 0.001, 1), 36, 1 / 64, 3),
+]
+
+
+@pytest.mark.parametrize("method", ["adi", "lod-ie", "lod-cn", "ts"])
+@pytest.mark.parametrize("mib", [1, 2])
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
+    """Beyond the sizes the reference can set up: every step against the oracle
+    restarted from the GPU's previous result (isolates per-step error)."""
+    surf, sargs, beta, n, dt, steps = CASES[ci]
+    print(f"case {surf} n={n} {method} mib{mib}", flush=True)
+    phys = mts.physics(*beta)
+    st = mts.Setup(mts.grid(n + 1), mts.surface(surf, *sargs), phys, mib=mib)
+    op = oracle.OracleProblem.from_setup(st, phys.beta_kind, beta[1:], -1.0, 1.0)
+    inst = mts.Instance(ctx, st, method, dt)
+    inst.initialize(0.0)
+    u = inst.download()
+    np.testing.assert_array_equal(u, op.init(0.0))
+    t = 0.0
+    for s in range(steps):
+        uo, _ = op.run(method, dt, t, 1, u)
+        t += dt
+        inst.step(t)
+        ug = inst.download()
+        assert rel_err(ug, uo) <= growth_tol(uo), f"step {s + 1}"
+        u = ug
+
+
+@pytest.mark.parametrize("n", [2, 3, 4, 5, 17, 64, 65, 128, 129, 257, 513, 1024])
+def test_gpu_batched_tdma(oracle, ctx, n):
+    rng = np.random.default_rng(n)
+    nl = 37
+    a = rng.uniform(-1, 1, (nl, n))
+    c = rng.uniform(-1, 1, (nl, n))
+    b = 2.2 + np.abs(a) + np.abs(c)
+    r = rng.uniform(-1, 1, (nl, n))
+    u = ctx.tdma_batched(a, b, c, r)
+    for l in range(nl):
+        ref = oracle.tdma(a[l], b[l], c[l], r[l])
+        assert rel_err(u[l], ref) < 1e-13
+
+
+def test_gpu_error_norms_match_oracle(mts, oracle, ctx):
+    phys = mts.physics("sincos", 10, 1)
+    st = mts.Setup(mts.grid(41), mts.surface("ellipsoid", 0.7, 0.5, 0.4, 1 - 2 ** -20), phys,
+                   mib=2)
+    op = oracle.OracleProblem.from_setup(st, 1, (10, 1), -1.0, 1.0)
+    inst = mts.Instance(ctx, st, "lod-cn", 0.05)
+    inst.initialize(0.0)
+    inst.run(0.0, 0.2)
+    u = inst.download()
+    li, l2, am = inst.error(0.2)
+    lo, l2o, amo = op.norms(u, 0.2)
+    assert abs(li - lo) <= 1e-15 * max(1.0, lo)
+    assert abs(l2 - l2o) <= 1e-13 * l2o
+    assert am == amo
+
+
+def test_gpu_layouts_round_trip(mts, ctx):
+    phys = mts.physics("const", 10, 1)
+    st = mts.Setup(mts.grid((17, 19, 21)), mts.surface("ellipsoid", 1, 1, 1, 0.47), phys, mib=1)
+    inst = mts.Instance(ctx, st, "adi", 0.1)
+    rng = np.random.default_rng(3)
+    u = rng.standard_normal(inst.shape)
+    inst.upload(u)
+    np.testing.assert_array_equal(inst.download(), u)
+    ref = np.ascontiguousarray(np.transpose(u, (2, 1, 0)))  # [ix][iy][iz]
+    np.testing.assert_array_equal(inst.download(layout=mts.LAYOUT_REF), ref)
+    inst.upload(ref, layout=mts.LAYOUT_REF)
+    np.testing.assert_array_equal(inst.download(), u)
