@@ -1,0 +1,232 @@
+// CPU bounds check of the per-step kernels' memory accesses.
+//
+// Builds the device table for one geometry exactly as an instance does, then
+// runs every global-memory and LDS access of one time step of the chosen
+// method on the host, using the same per-node / per-lane functions the HIP
+// kernels call (device_core.hpp), over buffers of exactly the device sizes.
+// Compiled with AddressSanitizer, so any index a kernel would form outside
+// its buffer aborts here instead of faulting a GPU.  The tridiagonal solve
+// itself touches registers only and is not emulated (rows keep their RHS).
+//
+//   mts_cpucheck <method adi|lod-ie|lod-cn|ts> <mib 1|2> <nodes per axis>
+//                <surface ellipsoid|torus|bumpy> <p0> <p1> [p2 p3]
+//                <beta const|sincos> <b0> <b1>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "device_core.hpp"
+#include "host_setup.hpp"
+#include "host_table.hpp"
+
+using namespace mts;
+
+namespace {
+
+struct Emu {
+  const SetupResult *s = nullptr;
+  DeviceTableHost t;
+  int px = 0;
+  size_t fe = 0;
+  std::vector<double> u, r;
+  std::vector<int8_t> ind;
+  DevGeom g;
+  DevMib mib[3];
+  PhysConst phys;
+};
+
+template <int DIR, int MMAX, int FLAGS>
+void emulate_line(Emu &E, const LineArgs &p) {
+  constexpr int RHS = FLAGS & 3;
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  constexpr bool HAS_S = (RHS == kRhsDouglas);
+  constexpr bool TWO = HAS_S || !SOLVE;
+  const int n = p.g.n[DIR];
+  const int ns = n + 1;
+  const int nj = LineMap<DIR>::nj(p.g), nk = LineMap<DIR>::nk(p.g);
+  const int ntile = (nj + kTile - 1) / kTile;
+  for (int bx = 0; bx < ntile; bx++)
+    for (int k = 1; k <= nk - 2; k++) {
+      // exact-size LDS tiles
+      std::unique_ptr<double[]> tA(new double[kTile * ns]);
+      std::unique_ptr<double[]> tB(new double[kTile * ns]);
+      std::unique_ptr<int8_t[]> tI(new int8_t[kTile * ns]);
+      std::memset(tA.get(), 0, sizeof(double) * kTile * ns);
+      std::memset(tB.get(), 0, sizeof(double) * kTile * ns);
+      std::memset(tI.get(), 1, kTile * ns);
+      const int j0 = bx * kTile;
+      for (int e = 0; e < kTile * n; e++) tile_load<DIR, HAS_S>(p, j0, k, e, tA.get(), tB.get(), tI.get());
+      for (int l = 0; l < kTile; l++) {
+        const int j = j0 + l;
+        if (j < 1 || j > nj - 2) continue;
+        for (int lane = 0; lane < 64; lane++) {
+          int s, m;
+          segment(n, lane, s, m);
+          double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+          line_rows<DIR, MMAX, FLAGS>(p, j, k, tA.get() + l * ns, tB.get() + l * ns,
+                                      tI.get() + l * ns, s, m, a, b, c, d);
+          double xs = m > 0 ? d[0] : 0.0, xe = 0.0;
+          for (int q = 0; q < MMAX; q++)
+            if (q == m - 1) xe = d[q];
+          line_write<DIR, MMAX, FLAGS>(p, j, k, tI.get() + l * ns, s, m, d, xs, xe,
+                                       (TWO && !HAS_S ? tB.get() : tA.get()) + l * ns);
+        }
+      }
+      const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
+      for (int e = 0; e < kTile * n; e++) tile_store<DIR>(p, j0, k, e, tO);
+    }
+}
+
+template <int FLAGS>
+void line(Emu &E, int dir, const LineArgs &p) {
+  int n = p.g.n[dir];
+  int pl = n / 2 > 64 ? 64 : n / 2;
+  int mmax = (n + pl - 1) / pl;
+  auto run = [&](auto mm) {
+    constexpr int MM = decltype(mm)::value;
+    if (dir == 0) emulate_line<0, MM, FLAGS>(E, p);
+    else if (dir == 1) emulate_line<1, MM, FLAGS>(E, p);
+    else emulate_line<2, MM, FLAGS>(E, p);
+  };
+  if (mmax <= 4) run(std::integral_constant<int, 4>());
+  else if (mmax <= 8) run(std::integral_constant<int, 8>());
+  else run(std::integral_constant<int, 16>());
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  if (argc < 9) {
+    std::fprintf(stderr, "usage: mts_cpucheck method mib nodes surface p.. beta b0 b1\n");
+    return 2;
+  }
+  std::string method = argv[1];
+  int mib = std::atoi(argv[2]);
+  int nodes = std::atoi(argv[3]);
+  std::string sk = argv[4];
+  int ai = 5;
+  std::unique_ptr<Surface> surf;
+  if (sk == "ellipsoid") {
+    surf.reset(new SurfaceEllipsoid(atof(argv[5]), atof(argv[6]), atof(argv[7]), atof(argv[8])));
+    ai = 9;
+  } else if (sk == "torus") {
+    surf.reset(new SurfaceTorus(atof(argv[5]), atof(argv[6])));
+    ai = 7;
+  } else if (sk == "bumpy") {
+    surf.reset(new SurfaceBumpy(atof(argv[5]), atof(argv[6]), atof(argv[7]), atof(argv[8])));
+    ai = 9;
+  } else {
+    return 2;
+  }
+  BetaModel beta;
+  beta.kind = std::string(argv[ai]) == "sincos" ? 1 : 0;
+  beta.p[0] = atof(argv[ai + 1]);
+  beta.p[1] = atof(argv[ai + 2]);
+  double lo[3] = {-1, -1, -1}, hi[3] = {1, 1, 1};
+  int nn[3] = {nodes, nodes, nodes};
+  Grid g = make_grid(lo, hi, nn);
+  SetupResult res;
+  BuildOptions opt;
+  opt.mib = mib;
+  std::string err = build_setup(g, *surf, beta, opt, res);
+  if (!err.empty()) {
+    std::printf("setup refused: %s\n", err.c_str());
+    return 3;
+  }
+  const double dt = 2.0 / (nodes - 1);
+  const bool adi = method == "adi";
+  const double tdt = (adi || method == "lod-ie") ? dt : 0.5 * dt;
+  Emu E;
+  E.s = &res;
+  build_device_table(res, beta, tdt, adi, E.t);
+  E.px = (nodes + 15) / 16 * 16;
+  E.fe = (size_t)E.px * nodes * nodes;
+  E.u.assign(E.fe, 0.25);
+  E.r.assign(E.fe, 0.0);
+  E.ind.assign(E.fe, 1);
+  for (int iz = 0; iz < nodes; iz++)
+    for (int iy = 0; iy < nodes; iy++)
+      std::memcpy(&E.ind[((size_t)iz * nodes + iy) * E.px], &res.ind[g.idx(0, iy, iz)], nodes);
+  for (int a = 0; a < 3; a++) {
+    E.g.n[a] = nodes;
+    E.g.tab[a] = E.t.tab[a].data();
+  }
+  E.g.px = E.px;
+  E.g.ind = E.ind.data();
+  for (int d = 0; d < 3; d++) {
+    E.mib[d].blk_ptr = E.t.blk_ptr[d].data();
+    E.mib[d].hdr = E.t.blk_hdr[d].data();
+    E.mib[d].val = E.t.blk_val[d].data();
+    E.mib[d].cr = E.t.cr_val[d].data();
+  }
+  LineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = E.g;
+  a.P = E.t.phys;
+  a.theta_dt = tdt;
+  a.dt = dt;
+  a.e_out = a.e_in = a.e_jump = a.e_f1 = a.e_f2 = 0.9;
+  a.c_f1 = a.c_f2 = 0.5 * dt;
+  if (adi) {
+    Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, 0.9};
+    for (int iz = 0; iz < nodes; iz++)
+      for (int iy = 0; iy < nodes; iy++)
+        for (int ix = 0; ix < nodes; ix++) rhs1_node(r1, ix, iy, iz);
+    SpecArgs sp;
+    sp.g = E.g;
+    sp.P = E.t.phys;
+    for (int d = 0; d < 3; d++) sp.mib[d] = E.mib[d];
+    sp.spec = E.t.spec.data();
+    sp.nspec = (int)(E.t.spec.size() / kSpecLen);
+    sp.u = E.u.data();
+    sp.R = E.r.data();
+    sp.dt = dt;
+    sp.e = 0.9;
+    for (int t = 0; t < sp.nspec; t++) spec_node(sp, t);
+    a.A = E.r.data();
+    a.out = E.r.data();
+    a.mib = E.mib[0];
+    line<kRhsCopy | kFlagSolve>(E, 0, a);
+    a.S = E.u.data();
+    a.mib = E.mib[1];
+    line<kRhsDouglas | kFlagSolve>(E, 1, a);
+    a.mib = E.mib[2];
+    a.out = E.u.data();
+    line<kRhsDouglas | kFlagSolve>(E, 2, a);
+  } else if (method == "lod-ie" || method == "lod-cn") {
+    a.A = a.out = E.u.data();
+    const bool cn = method == "lod-cn";
+    for (int d = 0; d < 3; d++) {
+      a.mib = E.mib[d];
+      a.in_analytic = d > 0;
+      if (cn) {
+        if (d < 2) line<kRhsCN | kFlagSolve | kFlagLhsJump>(E, d, a);
+        else line<kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF>(E, d, a);
+      } else {
+        if (d < 2) line<kRhsCopy | kFlagSolve | kFlagLhsJump>(E, d, a);
+        else line<kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF>(E, d, a);
+      }
+    }
+  } else if (method == "ts") {
+    a.A = a.out = E.u.data();
+    a.mib = E.mib[0];
+    line<kRhsCN>(E, 0, a);
+    a.in_analytic = 1;
+    a.mib = E.mib[1];
+    line<kRhsCN>(E, 1, a);
+    a.mib = E.mib[2];
+    line<kRhsTSZ | kFlagSolve | kFlagLhsJump>(E, 2, a);
+    a.mib = E.mib[1];
+    line<kRhsCopy | kFlagSolve | kFlagLhsJump>(E, 1, a);
+    a.mib = E.mib[0];
+    line<kRhsCopy | kFlagSolve | kFlagLhsJump>(E, 0, a);
+  } else {
+    return 2;
+  }
+  std::printf("ok %s mib%d %d^3 crossings %zu %zu %zu\n", method.c_str(), mib, nodes,
+              res.cross[0].size(), res.cross[1].size(), res.cross[2].size());
+  return 0;
+}

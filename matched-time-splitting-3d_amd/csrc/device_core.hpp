@@ -1,0 +1,530 @@
+// Per-node device functions of the per-step path, compiled as HIP device
+// code by kernels.hip and as plain host C++ by the CPU bounds-check build
+// (cpu_check.cpp), so every index a kernel forms can be checked on the host.
+#pragma once
+
+#include <cmath>
+#include <cstddef>
+
+#include "device_types.hpp"
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define MTS_HD __host__ __device__ __forceinline__
+#else
+#define MTS_HD inline
+#endif
+
+namespace mts {
+
+// ------------------------------------------------------------ geometry
+MTS_HD double tabv(const DevGeom &g, int a, int kind, int i) {
+  return g.tab[a][kind * g.n[a] + i];
+}
+MTS_HD size_t node_index(const DevGeom &g, int ix, int iy, int iz) {
+  return ((size_t)iz * g.n[1] + iy) * (size_t)g.px + ix;
+}
+
+// ------------------------------------------------------------- physics
+// u+ = e cos x cos y cos z (the boundary value and the outer exact solution)
+MTS_HD double outer_u(const DevGeom &g, double e, int ix, int iy, int iz) {
+  return e * tabv(g, 0, kTabCos, ix) * tabv(g, 1, kTabCos, iy) * tabv(g, 2, kTabCos, iz);
+}
+// u- = e sin x sin y sin z
+MTS_HD double inner_u(const DevGeom &g, double e, int ix, int iy, int iz) {
+  return e * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabSin, iy) * tabv(g, 2, kTabSin, iz);
+}
+
+// f = (3 beta - 1) u - grad(beta).grad(u) on the node's side.
+MTS_HD double source(const PhysConst &P, const DevGeom &g, double e,
+                                         bool outside, int ix, int iy, int iz) {
+  if (outside) return P.f_out * outer_u(g, e, ix, iy, iz);
+  double sx = tabv(g, 0, kTabSin, ix), sy = tabv(g, 1, kTabSin, iy), sz = tabv(g, 2, kTabSin, iz);
+  double u = e * sx * sy * sz;
+  if (P.beta_kind == 0) return P.f_in * u;
+  double cx = tabv(g, 0, kTabCos, ix), cy = tabv(g, 1, kTabCos, iy);
+  double b = P.b_in * (1.0 + 0.5 * sx * cy);
+  double bx = 0.5 * P.b_in * cx * cy;
+  double by = -0.5 * P.b_in * sx * sy;
+  double ux = e * cx * sy * sz;
+  double uy = e * sx * cy * sz;
+  return (3.0 * b - 1.0) * u - bx * ux - by * uy;
+}
+
+// Operator weights beta(x -+ h/2)/h^2 along axis d on the node's side.
+MTS_HD void op_weights(const PhysConst &P, const DevGeom &g, int d,
+                                           bool outside, int ix, int iy, int iz, double vd[3]) {
+  if (outside || P.beta_kind == 0) {
+    const int sd = outside ? 1 : 0;
+    vd[0] = P.vdc[d][sd][0];
+    vd[1] = P.vdc[d][sd][1];
+    vd[2] = P.vdc[d][sd][2];
+    return;
+  }
+  double b1, b2;
+  if (d == 0) {
+    double cy = tabv(g, 1, kTabCos, iy);
+    b1 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSinM, ix) * cy);
+    b2 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSinP, ix) * cy);
+  } else if (d == 1) {
+    double sx = tabv(g, 0, kTabSin, ix);
+    b1 = P.b_in * (1.0 + 0.5 * sx * tabv(g, 1, kTabCosM, iy));
+    b2 = P.b_in * (1.0 + 0.5 * sx * tabv(g, 1, kTabCosP, iy));
+  } else {
+    b1 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabCos, iy));
+    b2 = b1;
+  }
+  vd[0] = b1 / P.hh[d];
+  vd[1] = -(b1 + b2) / P.hh[d];
+  vd[2] = b2 / P.hh[d];
+}
+
+// [u] and u_dir = P^{-1}[d][0] n.[beta grad u] at one crossing
+// (Refresh_Jump, intersection.cc:2294-2490, approximate-jump mode).
+MTS_HD void jumps(const double *cr, double e, double &ju, double &jb) {
+  const double sx = cr[0], sy = cr[1], sz = cr[2], cx = cr[3], cy = cr[4], cz = cr[5];
+  const double bi = cr[6], bo = cr[7];
+  const double me = -e;
+  ju = e * cx * cy * cz - e * sx * sy * sz;
+  double jbx = bo * (me * sx * cy * cz) - bi * (e * cx * sy * sz);
+  double jby = bo * (me * cx * sy * cz) - bi * (e * sx * cy * sz);
+  double jbz = bo * (me * cx * cy * sz) - bi * (e * sx * sy * cz);
+  double xi = cr[8] * jbx + cr[9] * jby + cr[10] * jbz;
+  jb = cr[11] * xi;
+}
+
+// corner jump combination over components 5..8 of a weight row
+MTS_HD double jl(const double *w, double juL, double jbL, double juR,
+                                     double jbR) {
+  return juL * w[5] + jbL * w[6] + juR * w[7] + jbR * w[8];
+}
+
+// ------------------------------------------- MIB fictitious-point stencils
+// Sums of the patched rows without jump terms, in the reference's order
+// (douglas_adi.cc:116-195).  F(i) returns the field value at line row i.
+template <class F>
+MTS_HD double irr_sum0(const double *v, F f, int ll) {
+  const double *vd = v + kIrrVd, *wr = v + kIrrWr;
+  double s = vd[0] * f(ll - 1) + vd[1] * f(ll);
+  for (int i = 0; i < 4; i++) s += vd[2] * f(ll - 1 + i) * wr[i];
+  return s;
+}
+template <class F>
+MTS_HD double irr_sum1(const double *v, F f, int ll) {
+  const double *vd = v + kIrrVd + 3, *wl = v + kIrrWl;
+  double s = vd[1] * f(ll + 1) + vd[2] * f(ll + 2);
+  for (int i = 0; i < 4; i++) s += vd[0] * f(ll - 1 + i) * wl[i];
+  return s;
+}
+template <class F>
+MTS_HD double cor_sum0(const double *v, F f, int ll) {
+  const double *vd = v + kCorVd, *w = v + kCorLwr;
+  double s = vd[0] * f(ll - 1) + vd[1] * f(ll);
+  for (int i = 0; i < 5; i++) s += vd[2] * f(ll - 1 + i) * w[i];
+  return s;
+}
+template <class F>
+MTS_HD double cor_sum2(const double *v, F f, int ll) {
+  const double *vd = v + kCorVd + 6, *w = v + kCorRwl;
+  double s = vd[1] * f(ll + 2) + vd[2] * f(ll + 3);
+  for (int i = 0; i < 5; i++) s += vd[0] * f(ll - 1 + i) * w[i];
+  return s;
+}
+// middle corner row; with jumps interleaved as the Crank-Nicolson operators do
+// (lod.cc:414-440) when cn is set, else the Douglas order.
+template <class F>
+MTS_HD double cor_sum1(const double *v, F f, int ll, bool cn, double jL,
+                                           double jR) {
+  const double *vd = v + kCorVd + 3, *wl = v + kCorLwl, *wr = v + kCorRwr;
+  double s = 0.0;
+  for (int i = 0; i < 5; i++) s += vd[0] * f(ll - 1 + i) * wl[i];
+  if (cn) s += vd[0] * jL;
+  s += vd[1] * f(ll + 1);
+  for (int i = 0; i < 5; i++) s += vd[2] * f(ll - 1 + i) * wr[i];
+  if (cn) s += vd[2] * jR;
+  return s;
+}
+
+// segment of lane for a line of n rows: p lanes active, rows balanced
+MTS_HD void segment(int n, int lane, int &s, int &m) {
+  int p = n / 2;
+  if (p > 64) p = 64;
+  int q = n / p, rem = n - q * p;
+  if (lane < p) {
+    m = q + (lane < rem ? 1 : 0);
+    s = lane * q + (lane < rem ? lane : rem);
+  } else {
+    m = 0;
+    s = n;
+  }
+}
+
+// src += jump terms of block row rr (Src_2nd, douglas_adi.cc:1123-1428)
+MTS_HD double src_jump(const DevMib &mib, int bidx, int rr, double e,
+                                           double src) {
+  const BlockHdr h = mib.hdr[bidx];
+  const double *v = mib.val + h.off;
+  double ju, jb;
+  jumps(mib.cr + (size_t)h.c0 * kCrLen, e, ju, jb);
+  if (h.type == kBlkIrr) {
+    if (rr == 0) {
+      const double *w = v + kIrrWr;
+      src += v[kIrrVd + 2] * (ju * w[4] + jb * w[5]);
+    } else {
+      const double *w = v + kIrrWl;
+      src += v[kIrrVd + 3] * (ju * w[4] + jb * w[5]);
+    }
+    return src;
+  }
+  double juR, jbR;
+  jumps(mib.cr + (size_t)(h.c0 + 1) * kCrLen, e, juR, jbR);
+  if (rr == 0) {
+    src += v[kCorVd + 2] * jl(v + kCorLwr, ju, jb, juR, jbR);
+  } else if (rr == 1) {
+    src += v[kCorVd + 3] * jl(v + kCorLwl, ju, jb, juR, jbR);
+    src += v[kCorVd + 5] * jl(v + kCorRwr, ju, jb, juR, jbR);
+  } else {
+    src += v[kCorVd + 6] * jl(v + kCorRwl, ju, jb, juR, jbR);
+  }
+  return src;
+}
+
+// fictitious-point stencil sum at block row rr, reading the field from global memory
+MTS_HD double block_sum(const DevMib &mib, int bidx, int rr,
+                                            const double *u, size_t base, size_t rs) {
+  const BlockHdr h = mib.hdr[bidx];
+  const double *v = mib.val + h.off;
+  auto F = [&](int i) { return u[base + (size_t)i * rs]; };
+  if (h.type == kBlkIrr) return rr == 0 ? irr_sum0(v, F, h.ll) : irr_sum1(v, F, h.ll);
+  if (rr == 0) return cor_sum0(v, F, h.ll);
+  if (rr == 1) return cor_sum1(v, F, h.ll, false, 0.0, 0.0);
+  return cor_sum2(v, F, h.ll);
+}
+
+// Douglas stage-1 right-hand side at one node (douglas_adi.cc:93-359):
+// R = dt L_y u + dt L_z u + u + dt f, boundary nodes get the boundary value.
+MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
+  const DevGeom &g = p.g;
+  const size_t c = node_index(g, ix, iy, iz);
+  if (ix == 0 || iy == 0 || iz == 0 || ix == g.n[0] - 1 || iy == g.n[1] - 1 ||
+      iz == g.n[2] - 1) {
+    p.R[c] = outer_u(g, p.e, ix, iy, iz);
+    return;
+  }
+  const bool out = g.ind[c] > 0;
+  const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
+  double vy[3], vz[3];
+  op_weights(p.P, g, 1, out, ix, iy, iz, vy);
+  op_weights(p.P, g, 2, out, ix, iy, iz, vz);
+  const double u = p.u[c];
+  double sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
+  double sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+  double r = p.dt * sum_y;
+  r += p.dt * sum_z;
+  r += u;
+  r += p.dt * source(p.P, g, p.e, out, ix, iy, iz);
+  p.R[c] = r;
+}
+
+// Douglas stage-1 right-hand side at the t-th MIB-affected node: the
+// fictitious-point stencils along y and z and the source jump terms of all
+// three directions (douglas_adi.cc:116-312, Src_2nd :1158-1428).
+MTS_HD void spec_node(const SpecArgs &p, int t) {
+  const DevGeom &g = p.g;
+  const int *rec = p.spec + (size_t)t * kSpecLen;
+  const int ix = rec[0], iy = rec[1], iz = rec[2];
+  const size_t c = node_index(g, ix, iy, iz);
+  const bool out = g.ind[c] > 0;
+  const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
+  const double u = p.u[c];
+  double sum_y, sum_z;
+  if (rec[3 + 1] >= 0) {
+    sum_y = block_sum(p.mib[1], rec[3 + 1], rec[6 + 1], p.u, c - (size_t)iy * sy, sy);
+  } else {
+    double vy[3];
+    op_weights(p.P, g, 1, out, ix, iy, iz, vy);
+    sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
+  }
+  if (rec[3 + 2] >= 0) {
+    sum_z = block_sum(p.mib[2], rec[3 + 2], rec[6 + 2], p.u, c - (size_t)iz * sz, sz);
+  } else {
+    double vz[3];
+    op_weights(p.P, g, 2, out, ix, iy, iz, vz);
+    sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+  }
+  double src = source(p.P, g, p.e, out, ix, iy, iz);
+  for (int d = 0; d < 3; d++)
+    if (rec[3 + d] >= 0) src = src_jump(p.mib[d], rec[3 + d], rec[6 + d], p.e, src);
+  double r = p.dt * sum_y;
+  r += p.dt * sum_z;
+  r += u;
+  r += p.dt * src;
+  p.R[c] = r;
+}
+
+// ------------------------------------------------------- line stages
+constexpr int kTile = 16;  // lines per workgroup tile
+
+template <int DIR>
+struct LineMap {
+  // number of lines along the tile index j and the outer index k
+  MTS_HD static int nj(const DevGeom &g) { return DIR == 0 ? g.n[1] : g.n[0]; }
+  MTS_HD static int nk(const DevGeom &g) { return DIR == 2 ? g.n[1] : g.n[2]; }
+  MTS_HD static size_t stride(const DevGeom &g) {
+    return DIR == 0 ? 1 : DIR == 1 ? (size_t)g.px : (size_t)g.n[1] * g.px;
+  }
+  MTS_HD static size_t base(const DevGeom &g, int j, int k) {
+    return DIR == 0 ? ((size_t)k * g.n[1] + j) * g.px
+                    : DIR == 1 ? (size_t)k * g.n[1] * g.px + j : (size_t)k * g.px + j;
+  }
+  MTS_HD static void node(int j, int k, int i, int &ix, int &iy, int &iz) {
+    if (DIR == 0) { ix = i; iy = j; iz = k; }
+    else if (DIR == 1) { ix = j; iy = i; iz = k; }
+    else { ix = j; iy = k; iz = i; }
+  }
+};
+
+// tile element e -> (line l, row r): x lines are contiguous rows, y/z lines
+// put 16 consecutive lines (one 128-byte segment) next to each other
+template <int DIR>
+MTS_HD void tile_elem(int e, int n, int &l, int &r) {
+  if (DIR == 0) {
+    l = e / n;
+    r = e - l * n;
+  } else {
+    l = e & (kTile - 1);
+    r = e / kTile;
+  }
+}
+
+// Stage one element of the tile's input field(s) and classification in LDS.
+template <int DIR, bool HAS_S>
+MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, double *tA, double *tB,
+                      int8_t *tI) {
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const int n = g.n[DIR];
+  const int ns = n + 1;
+  int l, r;
+  tile_elem<DIR>(e, n, l, r);
+  const int j = j0 + l;
+  if (j >= M::nj(g)) return;
+  const size_t gi = M::base(g, j, k) + (size_t)r * M::stride(g);
+  double va = p.A[gi];
+  if (p.in_analytic && (r == 0 || r == n - 1)) {
+    int ix, iy, iz;
+    M::node(j, k, r, ix, iy, iz);
+    va = outer_u(g, p.e_in, ix, iy, iz);
+  }
+  tA[l * ns + r] = va;
+  if (HAS_S) tB[l * ns + r] = p.S[gi];
+  tI[l * ns + r] = g.ind[gi];
+}
+
+// Write one element of the tile's result back (interior lines only).
+template <int DIR>
+MTS_HD void tile_store(const LineArgs &p, int j0, int k, int e, const double *tO) {
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const int n = g.n[DIR];
+  int l, r;
+  tile_elem<DIR>(e, n, l, r);
+  const int j = j0 + l;
+  if (j < 1 || j > M::nj(g) - 2) return;
+  p.out[M::base(g, j, k) + (size_t)r * M::stride(g)] = tO[l * (n + 1) + r];
+}
+
+// Rows [s, s + m) of line (j, k): tridiagonal coefficients and right-hand
+// side of the stage, including the line's MIB blocks (the reference's
+// per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).
+// la/lb/li: the line's input field(s) and classification staged in LDS.
+template <int DIR, int MMAX, int FLAGS>
+MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const double *lb,
+                      const int8_t *li, int s, int m, double (&a)[MMAX], double (&b)[MMAX],
+                      double (&c)[MMAX], double (&d)[MMAX]) {
+  constexpr int RHS = FLAGS & 3;
+  constexpr bool LJ = (FLAGS & kFlagLhsJump) != 0;
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const int n = g.n[DIR];
+  const int nj = M::nj(g);
+  const double tdt = p.theta_dt;
+  const double mtdt = -tdt;
+  auto FA = [&](int i) { return la[i]; };
+  auto FS = [&](int i) { return lb[i]; };
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      a[r] = 0.0;
+      b[r] = 1.0;
+      c[r] = 0.0;
+      d[r] = 0.0;
+      if (r < m) {
+        const int i = s + r;
+        int ix, iy, iz;
+        M::node(j, k, i, ix, iy, iz);
+        if (i == 0 || i == n - 1) {
+          d[r] = outer_u(g, p.e_out, ix, iy, iz);
+        } else {
+          const bool out = li[i] > 0;
+          double vd[3];
+          op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
+          a[r] = mtdt * vd[0];
+          b[r] = 1 - tdt * vd[1];
+          c[r] = mtdt * vd[2];
+          if (RHS == kRhsCopy) {
+            d[r] = la[i];
+          } else if (RHS == kRhsDouglas) {
+            double sum = lb[i - 1] * vd[0] + lb[i] * vd[1] + lb[i + 1] * vd[2];
+            d[r] = -p.dt * sum + la[i];
+          } else {
+            double sum = la[i - 1] * vd[0] + la[i] * vd[1] + la[i + 1] * vd[2];
+            double v = tdt * sum + la[i];
+            if (RHS == kRhsTSZ) {
+              v = v + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+              v = v + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+            }
+            d[r] = v;
+          }
+        }
+      }
+    }
+
+    // ---- MIB blocks of this line
+    const int lid = k * nj + j;
+    const int bb = p.mib.blk_ptr[lid], be = p.mib.blk_ptr[lid + 1];
+    for (int bi = bb; bi < be; bi++) {
+      const BlockHdr h = p.mib.hdr[bi];
+      const int ll = h.ll;
+      const int nr = (h.type == kBlkIrr) ? 2 : 3;
+      if (m == 0 || ll + nr - 1 < s || ll >= s + m) continue;
+      const double *v = p.mib.val + h.off;
+      double ju = 0.0, jb = 0.0, juR = 0.0, jbR = 0.0;
+      if (RHS == kRhsCN || RHS == kRhsTSZ || LJ) {
+        jumps(p.mib.cr + (size_t)h.c0 * kCrLen, p.e_jump, ju, jb);
+        if (h.type == kBlkCor) jumps(p.mib.cr + (size_t)(h.c0 + 1) * kCrLen, p.e_jump, juR, jbR);
+      }
+      double bv[3];
+      const double *lhs;
+      if (h.type == kBlkIrr) {
+        const double *vd0 = v + kIrrVd, *vd1 = v + kIrrVd + 3;
+        const double *wl = v + kIrrWl, *wr = v + kIrrWr;
+        const double J0 = ju * wr[4] + jb * wr[5];
+        const double J1 = ju * wl[4] + jb * wl[5];
+        if (RHS == kRhsCopy) {
+          bv[0] = la[ll];
+          bv[1] = la[ll + 1];
+        } else if (RHS == kRhsDouglas) {
+          bv[0] = -p.dt * irr_sum0(v, FS, ll) + la[ll];
+          bv[1] = -p.dt * irr_sum1(v, FS, ll) + la[ll + 1];
+        } else {
+          double s0 = irr_sum0(v, FA, ll) + vd0[2] * J0;
+          double s1 = irr_sum1(v, FA, ll) + vd1[0] * J1;
+          bv[0] = tdt * s0 + la[ll];
+          bv[1] = tdt * s1 + la[ll + 1];
+        }
+        if (RHS == kRhsTSZ) {
+          for (int r = 0; r < 2; r++) {
+            int ix, iy, iz;
+            M::node(j, k, ll + r, ix, iy, iz);
+            const bool out = li[ll + r] > 0;
+            bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+            bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+          }
+        }
+        if (LJ) {
+          bv[0] = bv[0] + tdt * vd0[2] * J0;
+          bv[1] = bv[1] + tdt * vd1[0] * J1;
+        }
+        if (SOLVE) {  // Convert2Tri row reduction, only where the rows are solved
+          const double *mu = v + kIrrMul;
+          bv[1] -= mu[0] * bv[0];
+          bv[0] -= mu[1] * bv[1];
+        }
+        lhs = v + kIrrLhs;
+      } else {
+        const double *vd0 = v + kCorVd, *vd1 = v + kCorVd + 3, *vd2 = v + kCorVd + 6;
+        const double JLwr = jl(v + kCorLwr, ju, jb, juR, jbR);
+        const double JLwl = jl(v + kCorLwl, ju, jb, juR, jbR);
+        const double JRwr = jl(v + kCorRwr, ju, jb, juR, jbR);
+        const double JRwl = jl(v + kCorRwl, ju, jb, juR, jbR);
+        if (RHS == kRhsCopy) {
+          bv[0] = la[ll];
+          bv[1] = la[ll + 1];
+          bv[2] = la[ll + 2];
+        } else if (RHS == kRhsDouglas) {
+          bv[0] = -p.dt * cor_sum0(v, FS, ll) + la[ll];
+          bv[1] = -p.dt * cor_sum1(v, FS, ll, false, 0.0, 0.0) + la[ll + 1];
+          bv[2] = -p.dt * cor_sum2(v, FS, ll) + la[ll + 2];
+        } else {
+          double s0 = cor_sum0(v, FA, ll) + vd0[2] * JLwr;
+          double s1 = cor_sum1(v, FA, ll, true, JLwl, JRwr);
+          double s2 = cor_sum2(v, FA, ll) + vd2[0] * JRwl;
+          bv[0] = tdt * s0 + la[ll];
+          bv[1] = tdt * s1 + la[ll + 1];
+          bv[2] = tdt * s2 + la[ll + 2];
+        }
+        if (RHS == kRhsTSZ) {
+          for (int r = 0; r < 3; r++) {
+            int ix, iy, iz;
+            M::node(j, k, ll + r, ix, iy, iz);
+            const bool out = li[ll + r] > 0;
+            bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+            bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+          }
+        }
+        if (LJ) {
+          bv[0] = bv[0] + tdt * vd0[2] * JLwr;
+          bv[1] = bv[1] + tdt * vd1[0] * JLwl;
+          bv[1] += tdt * vd1[2] * JRwr;
+          bv[2] = bv[2] + tdt * vd2[0] * JRwl;
+        }
+        if (SOLVE) {
+          const double *mu = v + kCorMul;
+          bv[1] -= mu[0] * bv[0];
+          bv[2] -= mu[1] * bv[0];
+          bv[0] -= mu[2] * bv[2];
+          bv[1] -= mu[3] * bv[2];
+          bv[2] -= mu[4] * bv[1];
+          bv[0] -= mu[5] * bv[1];
+        }
+        lhs = v + kCorLhs;
+      }
+#pragma unroll
+      for (int r = 0; r < MMAX; r++) {
+        const int rr = s + r - ll;
+        if (r < m && rr >= 0 && rr < nr) {
+          if (SOLVE) {
+            a[r] = lhs[rr * 3 + 0];
+            b[r] = lhs[rr * 3 + 1];
+            c[r] = lhs[rr * 3 + 2];
+          }
+          d[r] = rr == 0 ? bv[0] : rr == 1 ? bv[1] : bv[2];
+        }
+      }
+    }
+
+}
+
+// Epilogue of lane rows and write to the LDS output tile.
+template <int DIR, int MMAX, int FLAGS>
+MTS_HD void line_write(const LineArgs &p, int j, int k, const int8_t *li, int s, int m,
+                       const double (&d)[MMAX], double xs, double xe, double *lo) {
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
+  using M = LineMap<DIR>;
+  for (int r = 0; r < MMAX; r++) {
+    if (r < m) {
+      const int i = s + r;
+      double x = SOLVE ? (r == 0 ? xs : (r == m - 1 ? xe : d[r])) : d[r];
+      if (EPI) {
+        int ix, iy, iz;
+        M::node(j, k, i, ix, iy, iz);
+        x = x + p.c_f1 * source(p.P, p.g, p.e_f1, li[i] > 0, ix, iy, iz);
+      }
+      lo[i] = x;
+    }
+  }
+}
+
+}  // namespace mts
