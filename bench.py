@@ -39,6 +39,9 @@ def main():
     ap.add_argument("--mib", type=int, default=2, choices=[1, 2])
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the checker restatement on the host (0 to skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain device copy of known size (the reference "
+                         "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -46,7 +49,8 @@ def main():
               "multi-GPU runs are not part of it (use --gpus 1)", file=sys.stderr)
         return 2
 
-    import numpy as np
+    if args.copy_calibration:
+        return copy_calibration()
 
     mts = load_mts()
     n = args.n + 1
@@ -152,6 +156,25 @@ def main():
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def copy_calibration():
+    """dst = src for 2^26 doubles (512 MiB read, 512 MiB written), three times:
+    the HBM counters of this kernel per dispatch over its known byte count give
+    the scale of FETCH_SIZE and WRITE_SIZE on this device."""
+    import torch
+
+    count = 1 << 26
+    src = torch.arange(count, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    ok = bool(torch.equal(dst, src))
+    print(json.dumps({"copy_calibration": {"bytes_read": count * 8, "bytes_written": count * 8,
+                                           "dispatches": 3, "ok": ok}}), flush=True)
+    return 0 if ok else 1
 
 
 def cpu_baseline(mts, st, method, dt, nodes, phys):

@@ -76,10 +76,12 @@ struct DevBuf {
     if (count == 0) return hipSuccess;
     return hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
   }
-  hipError_t upload(const T *h, size_t count) {
+  // Allocates and copies on stream st.  The copy is asynchronous: h must stay
+  // valid, and no other stream may read the buffer, until st is synchronized.
+  hipError_t upload(const T *h, size_t count, hipStream_t st) {
     hipError_t e = alloc(count);
     if (e != hipSuccess || count == 0) return e;
-    return hipMemcpy(p, h, count * sizeof(T), hipMemcpyHostToDevice);
+    return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st);
   }
 };
 
@@ -110,6 +112,7 @@ struct mts_instance {
   DevBuf<mts::BlockHdr> blk_hdr[3];
   DevBuf<double> blk_val[3];
   DevBuf<double> cr_val[3];
+  DevBuf<uint8_t> seq[3];
   DevBuf<int> spec;
   int nspec = 0;
   DevBuf<double> part_max, part_sq;
@@ -127,7 +130,10 @@ struct mts_instance {
   std::vector<double> stage_ms;
   std::vector<int> stage_n;
   ~mts_instance() {
-    if (stream) (void)hipStreamDestroy(stream);
+    if (stream) {
+      (void)hipStreamSynchronize(stream);  // pending copies may read host staging
+      (void)hipStreamDestroy(stream);
+    }
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
   int stage(const char *name) {
@@ -348,29 +354,34 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
   }
   in->phys = t.phys;
 
+  // Every fill and copy below goes on the instance's own stream, which is
+  // non-blocking: work on the legacy null stream (hipMemset, and the staged
+  // DMA of a pageable hipMemcpy, which may still be in flight when hipMemcpy
+  // returns) is not ordered before the kernels launched on it.  The host
+  // staging (t, ind) stays alive until the stream is synchronized at the end.
+  const hipStream_t st = in->stream;
   MTS_HIP(in->u.alloc(in->field_elems));
-  MTS_HIP(hipMemset(in->u.p, 0, in->field_elems * sizeof(double)));
+  MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
   if (method == MTS_ADI) {
     MTS_HIP(in->r.alloc(in->field_elems));
-    MTS_HIP(hipMemset(in->r.p, 0, in->field_elems * sizeof(double)));
+    MTS_HIP(hipMemsetAsync(in->r.p, 0, in->field_elems * sizeof(double), st));
   }
-  {
-    std::vector<int8_t> ind(in->field_elems, 1);
-    for (int iz = 0; iz < g.n[2]; iz++)
-      for (int iy = 0; iy < g.n[1]; iy++)
-        std::memcpy(&ind[((size_t)iz * g.n[1] + iy) * in->px], &setup->res.ind[g.idx(0, iy, iz)],
-                    g.n[0]);
-    MTS_HIP(in->ind.upload(ind.data(), ind.size()));
-  }
-  for (int a = 0; a < 3; a++) MTS_HIP(in->tab[a].upload(t.tab[a].data(), t.tab[a].size()));
+  std::vector<int8_t> ind(in->field_elems, 1);
+  for (int iz = 0; iz < g.n[2]; iz++)
+    for (int iy = 0; iy < g.n[1]; iy++)
+      std::memcpy(&ind[((size_t)iz * g.n[1] + iy) * in->px], &setup->res.ind[g.idx(0, iy, iz)],
+                  g.n[0]);
+  MTS_HIP(in->ind.upload(ind.data(), ind.size(), st));
+  for (int a = 0; a < 3; a++) MTS_HIP(in->tab[a].upload(t.tab[a].data(), t.tab[a].size(), st));
   for (int d = 0; d < 3; d++) {
-    MTS_HIP(in->blk_ptr[d].upload(t.blk_ptr[d].data(), t.blk_ptr[d].size()));
-    MTS_HIP(in->blk_hdr[d].upload(t.blk_hdr[d].data(), t.blk_hdr[d].size()));
-    MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size()));
-    MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size()));
+    MTS_HIP(in->blk_ptr[d].upload(t.blk_ptr[d].data(), t.blk_ptr[d].size(), st));
+    MTS_HIP(in->blk_hdr[d].upload(t.blk_hdr[d].data(), t.blk_hdr[d].size(), st));
+    MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size(), st));
+    MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size(), st));
+    if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
   }
   in->nspec = (int)(t.spec.size() / mts::kSpecLen);
-  MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size()));
+  MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size(), st));
   mts::DevGeom &dg = in->geom;
   for (int a = 0; a < 3; a++) {
     dg.n[a] = g.n[a];
@@ -383,11 +394,13 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
     in->mib[d].hdr = in->blk_hdr[d].p;
     in->mib[d].val = in->blk_val[d].p;
     in->mib[d].cr = in->cr_val[d].p;
+    in->mib[d].seq = in->seq[d].p;
   }
   int nb = mts::norms_blocks(dg);
   MTS_HIP(in->part_max.alloc(nb));
   MTS_HIP(in->part_sq.alloc(nb));
   MTS_HIP(in->part_idx.alloc(nb));
+  MTS_HIP(hipStreamSynchronize(st));
   *out = in.release();
   return MTS_OK;
 }
@@ -672,13 +685,13 @@ int mts_tdma_batched(mts_ctx *ctx, int n, int nlines, const double *a, const dou
   MTS_HIP(hipSetDevice(ctx->device));
   size_t cnt = (size_t)n * nlines;
   DevBuf<double> da, db, dc, dr, du;
-  MTS_HIP(da.upload(a, cnt));
-  MTS_HIP(db.upload(b, cnt));
-  MTS_HIP(dc.upload(c, cnt));
-  MTS_HIP(dr.upload(r, cnt));
+  // everything on the null stream, in order; the last copy is synchronous
+  MTS_HIP(da.upload(a, cnt, nullptr));
+  MTS_HIP(db.upload(b, cnt, nullptr));
+  MTS_HIP(dc.upload(c, cnt, nullptr));
+  MTS_HIP(dr.upload(r, cnt, nullptr));
   MTS_HIP(du.alloc(cnt));
   MTS_HIP(mts::launch_tdma(n, nlines, da.p, db.p, dc.p, dr.p, du.p, nullptr));
-  MTS_HIP(hipDeviceSynchronize());
   MTS_HIP(hipMemcpy(u, du.p, cnt * sizeof(double), hipMemcpyDeviceToHost));
   return MTS_OK;
 }

@@ -68,6 +68,10 @@ void emulate_line(Emu &E, const LineArgs &p) {
           double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
           line_rows<DIR, MMAX, FLAGS>(p, j, k, tA.get() + l * ns, tB.get() + l * ns,
                                       tI.get() + l * ns, s, m, a, b, c, d);
+          if ((FLAGS & kFlagSolve) && p.mib.seq) {  // the kernel's per-line solver choice
+            volatile uint8_t sq = p.mib.seq[k * nj + j];
+            (void)sq;
+          }
           double xs = m > 0 ? d[0] : 0.0, xe = 0.0;
           for (int q = 0; q < MMAX; q++)
             if (q == m - 1) xe = d[q];
@@ -161,6 +165,7 @@ int main(int argc, char **argv) {
     E.mib[d].hdr = E.t.blk_hdr[d].data();
     E.mib[d].val = E.t.blk_val[d].data();
     E.mib[d].cr = E.t.cr_val[d].data();
+    E.mib[d].seq = E.t.n_seq[d] > 0 ? E.t.seq[d].data() : nullptr;
   }
   LineArgs a;
   std::memset(&a, 0, sizeof(a));

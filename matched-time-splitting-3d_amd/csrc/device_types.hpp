@@ -66,11 +66,17 @@ struct DevGeom {
 };
 
 // Device view of one direction's MIB blocks.
+// seq: one byte per line, 1 where a patched row of the line is not
+// diagonally dominant (|b| < |a| + |c| after Convert2Tri).  Such lines are
+// solved with the sequential Thomas algorithm (the reference's TDMA order):
+// the partitioned solve is stable only for diagonally dominant lines.  Null
+// when no line of the direction needs it.
 struct DevMib {
   const int *blk_ptr;
   const BlockHdr *hdr;
   const double *val;
   const double *cr;
+  const uint8_t *seq;
 };
 
 // Right-hand-side kinds of a line stage.
@@ -82,6 +88,7 @@ enum : int {
   kFlagLhsJump = 4,   // patched LHS rows carry theta dt vd (jumps) on their RHS
   kFlagSolve = 8,     // tridiagonal solve along the line
   kFlagEpiF = 16,     // epilogue: x + dt f(t)  on every row (LOD last sweep)
+  kFlagSeq = 32,      // some lines need the sequential Thomas solve (DevMib::seq)
 };
 
 struct LineArgs {

@@ -47,6 +47,11 @@ void operator_weights(const SetupResult &s, const BetaModel &beta, int d, int ix
   vd[2] = b2 / (h * h);
 }
 
+namespace {
+// row (a, b, c) of a tridiagonal system is diagonally dominant
+bool dominant(double a, double b, double c) { return !(std::fabs(b) < std::fabs(a) + std::fabs(c)); }
+}  // namespace
+
 void build_device_table(const SetupResult &s, const BetaModel &beta, double theta_dt,
                         bool with_special, DeviceTableHost &t) {
   const Grid &g = s.grid;
@@ -58,6 +63,8 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
     const int nl = s.n_lines[d];
     const int nj = s.line_nj(d);
     t.blk_ptr[d].assign((size_t)nl + 1, 0);
+    t.seq[d].assign((size_t)nl, 0);
+    t.n_seq[d] = 0;
     t.blk_hdr[d].clear();
     t.blk_val[d].clear();
     t.cr_val[d].clear();
@@ -132,6 +139,7 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
           rec[kIrrLhs + 5] = r1[3];
           rec[kIrrMul + 0] = m1;
           rec[kIrrMul + 1] = m2;
+          if (!dominant(r0[0], r0[1], r0[2]) || !dominant(r1[1], r1[2], r1[3])) t.seq[d][lid] = 1;
           ci += 1;
         } else {
           const Crossing &R = s.cross[d][ci + 1];
@@ -183,6 +191,8 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
           rec[kCorMul + 3] = m12;
           rec[kCorMul + 4] = m21;
           rec[kCorMul + 5] = m01;
+          for (int r = 0; r < 3; r++)
+            if (!dominant(rows[r][r], rows[r][r + 1], rows[r][r + 2])) t.seq[d][lid] = 1;
           ci += 2;
         }
         int bidx = (int)t.blk_hdr[d].size();
@@ -210,6 +220,7 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
       }
     }
     t.blk_ptr[d][nl] = (int)t.blk_hdr[d].size();
+    for (int lid = 0; lid < nl; lid++) t.n_seq[d] += t.seq[d][lid];
     t.n_blocks[d] = (int)t.blk_hdr[d].size();
   }
 

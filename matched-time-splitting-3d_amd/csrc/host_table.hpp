@@ -13,6 +13,8 @@ struct DeviceTableHost {
   std::vector<BlockHdr> blk_hdr[3];
   std::vector<double> blk_val[3];
   std::vector<double> cr_val[3];    // kCrLen per crossing
+  std::vector<uint8_t> seq[3];      // per line: 1 = solve with sequential Thomas
+  int n_seq[3] = {0, 0, 0};
   std::vector<int> spec;            // kSpecLen per special node (Douglas stage 1)
   std::vector<double> tab[3];       // kTabCount * n[a] per axis
   PhysConst phys;

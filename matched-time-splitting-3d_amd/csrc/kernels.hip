@@ -147,6 +147,76 @@ __device__ __forceinline__ void wave_solve(double (&a)[MMAX], double (&b)[MMAX],
   }
 }
 
+__device__ __forceinline__ double read_lane(double v, int k) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
+  return __hiloint2double(hi, lo);
+}
+
+// The sequential Thomas algorithm over the whole line, in the reference's
+// TDMA operation order (douglas_adi.cc:1784-1807), for lines the partitioned
+// solve would not handle stably (a non-diagonally-dominant row).  The rows
+// stay distributed as for wave_solve; the recurrence runs wave-uniformly:
+// row i's coefficients are read from the lane that owns it, and its factor
+// gamma_i and forward value are written back into that lane's a and d.  On
+// return d holds the solution (xs, xe as for wave_solve).
+template <int MMAX>
+__device__ __forceinline__ void wave_thomas(double (&a)[MMAX], const double (&b)[MMAX],
+                                            const double (&c)[MMAX], double (&d)[MMAX], int n,
+                                            int m, int lane, double &xs, double &xe) {
+  const int p = n / 2 > 64 ? 64 : n / 2;
+  double bet = 1.0, uprev = 0.0, cprev = 0.0;
+  bool first = true;
+  for (int k = 0; k < p; k++) {
+    int sk, mk;
+    segment(n, k, sk, mk);
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      if (r < mk) {
+        const double ai = read_lane(a[r], k), bi = read_lane(b[r], k), di = read_lane(d[r], k);
+        double g = 0.0, u;
+        if (first) {
+          bet = bi;
+          u = di / bet;
+          first = false;
+        } else {
+          g = cprev / bet;
+          bet = bi - ai * g;
+          u = (di - ai * uprev) / bet;
+        }
+        if (lane == k) {
+          a[r] = g;
+          d[r] = u;
+        }
+        cprev = read_lane(c[r], k);
+        uprev = u;
+      }
+    }
+  }
+  double unext = 0.0, gnext = 0.0;
+  bool last = true;
+  for (int k = p - 1; k >= 0; k--) {
+    int sk, mk;
+    segment(n, k, sk, mk);
+#pragma unroll
+    for (int r = MMAX - 1; r >= 0; r--) {
+      if (r < mk) {
+        double u = read_lane(d[r], k);
+        if (!last) u -= gnext * unext;
+        last = false;
+        if (lane == k) d[r] = u;
+        gnext = read_lane(a[r], k);
+        unext = u;
+      }
+    }
+  }
+  xs = d[0];
+  xe = d[0];
+#pragma unroll
+  for (int r = 0; r < MMAX; r++)
+    if (r == m - 1) xe = d[r];
+}
+
 // value of local row r after wave_solve
 template <int MMAX>
 __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int m, double xs,
@@ -161,6 +231,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;  // second LDS tile: S, or the output of explicit stages
+  constexpr bool SEQ = SOLVE && (FLAGS & kFlagSeq) != 0;
   const DevGeom &g = p.g;
   const int n = g.n[DIR];
   const int ns = n + 1;
@@ -189,7 +260,12 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
     line_rows<DIR, MMAX, FLAGS>(p, j, k, tA + l * ns, tB + l * ns, li, s, m, a, b, c, d);
     double xs = 0.0, xe = 0.0;
-    if (SOLVE) wave_solve<MMAX>(a, b, c, d, m, lane, xs, xe);
+    if (SOLVE) {
+      if (SEQ && p.mib.seq[k * nj + j])  // uniform per wavefront
+        wave_thomas<MMAX>(a, b, c, d, n, m, lane, xs, xe);
+      else
+        wave_solve<MMAX>(a, b, c, d, m, lane, xs, xe);
+    }
     line_write<DIR, MMAX, FLAGS>(p, j, k, li, s, m, d, xs, xe,
                                  (TWO && !HAS_S ? tB : tA) + l * ns);
   }
@@ -374,24 +450,27 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
 }  // namespace
 
 // --------------------------------------------------------------- launchers
+// The sequential-solve variant is compiled separately so that its register
+// needs do not lower the occupancy of the common case (no such line).
+#define MTS_LINE_CASE(F)                                                       \
+  case (F):                                                                    \
+    return seq ? launch_line_f<(F) | kFlagSeq>(dir, a, s) : launch_line_f<(F)>(dir, a, s);
+
 hipError_t launch_line(int dir, int flags, const LineArgs &a, hipStream_t s) {
+  const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
   switch (flags) {
-    case kRhsCopy | kFlagSolve: return launch_line_f<kRhsCopy | kFlagSolve>(dir, a, s);
-    case kRhsCopy | kFlagSolve | kFlagLhsJump:
-      return launch_line_f<kRhsCopy | kFlagSolve | kFlagLhsJump>(dir, a, s);
-    case kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF:
-      return launch_line_f<kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF>(dir, a, s);
-    case kRhsDouglas | kFlagSolve: return launch_line_f<kRhsDouglas | kFlagSolve>(dir, a, s);
-    case kRhsCN | kFlagSolve | kFlagLhsJump:
-      return launch_line_f<kRhsCN | kFlagSolve | kFlagLhsJump>(dir, a, s);
-    case kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF:
-      return launch_line_f<kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF>(dir, a, s);
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF)
+    MTS_LINE_CASE(kRhsDouglas | kFlagSolve)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF)
+    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump)
     case kRhsCN: return launch_line_f<kRhsCN>(dir, a, s);
-    case kRhsTSZ | kFlagSolve | kFlagLhsJump:
-      return launch_line_f<kRhsTSZ | kFlagSolve | kFlagLhsJump>(dir, a, s);
     default: return hipErrorInvalidValue;
   }
 }
+#undef MTS_LINE_CASE
 
 hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s) {
   dim3 grid((a.g.n[0] + 255) / 256, a.g.n[1], a.g.n[2]);

@@ -170,6 +170,197 @@ int mts_oracle_tdma(int n, const double *a, const double *b, const double *c, co
   return 0;
 }
 
+/* ------------------------------------------------ alternative line solvers
+ * Not part of the reference.  They let the tests separate rounding from
+ * error: the same step is solved with a different elimination and the
+ * differences measured against the conditioning of the line systems.
+ *
+ * gtsv: Gaussian elimination with partial (row-interchange) pivoting for a
+ * tridiagonal matrix, the algorithm of LAPACK dgtsv. */
+static int gtsv(int n, const double *a, const double *b, const double *c, const double *r,
+                double *u) {
+  double *dl = (double *)malloc(sizeof(double) * (size_t)n * 4);
+  if (!dl) return -1;
+  double *dd = dl + n, *du = dd + n, *du2 = du + n;
+  for (int i = 0; i < n; i++) {
+    dl[i] = i + 1 < n ? a[i + 1] : 0.0; /* sub-diagonal entry of row i+1 */
+    dd[i] = b[i];
+    du[i] = c[i];
+    du2[i] = 0.0;
+    u[i] = r[i];
+  }
+  for (int i = 0; i < n - 1; i++) {
+    if (fabs(dd[i]) >= fabs(dl[i])) {
+      if (dd[i] == 0.0) {
+        free(dl);
+        return -1;
+      }
+      double f = dl[i] / dd[i];
+      dd[i + 1] -= f * du[i];
+      u[i + 1] -= f * u[i];
+      dl[i] = 0.0;
+    } else {
+      double f = dd[i] / dl[i];
+      dd[i] = dl[i];
+      double tmp = dd[i + 1];
+      dd[i + 1] = du[i] - f * tmp;
+      if (i < n - 2) {
+        dl[i] = du[i + 1];
+        du[i + 1] = -f * dl[i];
+      }
+      du2[i] = i < n - 2 ? dl[i] : 0.0;
+      du[i] = tmp;
+      tmp = u[i];
+      u[i] = u[i + 1];
+      u[i + 1] = tmp - f * u[i + 1];
+    }
+  }
+  if (dd[n - 1] == 0.0) {
+    free(dl);
+    return -1;
+  }
+  u[n - 1] /= dd[n - 1];
+  if (n > 1) u[n - 2] = (u[n - 2] - du[n - 2] * u[n - 1]) / dd[n - 2];
+  for (int i = n - 3; i >= 0; i--) u[i] = (u[i] - du[i] * u[i + 1] - du2[i] * u[i + 2]) / dd[i];
+  free(dl);
+  return 0;
+}
+
+/* Model of the GPU's line solve, operation for operation.  A line with a
+ * row that is not diagonally dominant (|b| < |a| + |c|) is solved with the
+ * Thomas algorithm.  Any other line: rows split into p = min(64, n/2)
+ * balanced segments over 64 lanes, each segment eliminated down and up
+ * keeping its first unknown as a parameter, folded to one equation per lane
+ * in the segment-end unknowns, parallel cyclic reduction over the 64 lanes
+ * (idle lanes are identity rows), then each segment back-substituted. */
+static void seg_of(int n, int lane, int *s, int *m) {
+  int p = n / 2;
+  if (p > 64) p = 64;
+  int q = n / p, rem = n - q * p;
+  if (lane < p) {
+    *m = q + (lane < rem ? 1 : 0);
+    *s = lane * q + (lane < rem ? lane : rem);
+  } else {
+    *m = 0;
+    *s = n;
+  }
+}
+
+static int part_solve(int n, const double *A, const double *B, const double *C, const double *R,
+                      double *U) {
+  enum { L = 64 };
+  for (int i = 0; i < n; i++)
+    if (fabs(B[i]) < fabs(A[i]) + fabs(C[i])) return mts_oracle_tdma(n, A, B, C, R, U);
+  double *w = (double *)malloc(sizeof(double) * (size_t)n * 3);
+  if (!w) return -1;
+  double *a = w, *c = w + n, *d = w + 2 * n;
+  double A1[L], B1[L], C1[L], D1[L], A2[L], B2[L], C2[L], D2[L];
+  double ea[L], eb[L], ec[L], ed[L], xs[L], xe[L];
+  int sv[L], mv[L];
+  for (int i = 0; i < n; i++) {
+    a[i] = A[i];
+    c[i] = C[i];
+    d[i] = R[i];
+  }
+  for (int ln = 0; ln < L; ln++) {
+    int s, m;
+    seg_of(n, ln, &s, &m);
+    sv[ln] = s;
+    mv[ln] = m;
+    A1[ln] = 0.0; B1[ln] = 1.0; C1[ln] = 0.0; D1[ln] = 0.0;
+    A2[ln] = 0.0; B2[ln] = 1.0; C2[ln] = 0.0; D2[ln] = 0.0;
+    if (m < 2) continue;
+    double *ra = a + s, *rc = c + s, *rd = d + s;
+    const double *rb = B + s;
+    double r = 1.0 / rb[1];
+    ra[1] = ra[1] * r;
+    rc[1] = rc[1] * r;
+    rd[1] = rd[1] * r;
+    A2[ln] = ra[1];
+    C2[ln] = rc[1];
+    D2[ln] = rd[1];
+    for (int i = 2; i < m; i++) {
+      r = 1.0 / (rb[i] - ra[i] * rc[i - 1]);
+      rd[i] = (rd[i] - ra[i] * rd[i - 1]) * r;
+      ra[i] = -ra[i] * ra[i - 1] * r;
+      rc[i] = rc[i] * r;
+      A2[ln] = ra[i];
+      C2[ln] = rc[i];
+      D2[ln] = rd[i];
+    }
+    for (int i = m - 3; i >= 1; i--) {
+      rd[i] = rd[i] - rc[i] * rd[i + 1];
+      ra[i] = ra[i] - rc[i] * ra[i + 1];
+      rc[i] = -rc[i] * rc[i + 1];
+    }
+    if (m == 2) {
+      A1[ln] = ra[0]; B1[ln] = rb[0]; C1[ln] = rc[0]; D1[ln] = rd[0];
+    } else {
+      A1[ln] = ra[0];
+      B1[ln] = rb[0] - rc[0] * ra[1];
+      C1[ln] = -rc[0] * rc[1];
+      D1[ln] = rd[0] - rc[0] * rd[1];
+    }
+  }
+  for (int ln = 0; ln < L; ln++) {
+    ea[ln] = 0.0; eb[ln] = 1.0; ec[ln] = 0.0; ed[ln] = 0.0;
+    if (mv[ln] < 2) continue;
+    double a1n = 0.0, b1n = 1.0, c1n = 0.0, d1n = 0.0;
+    if (ln + 1 < L) {
+      a1n = A1[ln + 1]; b1n = B1[ln + 1]; c1n = C1[ln + 1]; d1n = D1[ln + 1];
+    }
+    double i1 = 1.0 / B1[ln], in = 1.0 / b1n;
+    ea[ln] = -A2[ln] * A1[ln] * i1;
+    eb[ln] = B2[ln] - A2[ln] * C1[ln] * i1 - C2[ln] * a1n * in;
+    ec[ln] = -C2[ln] * c1n * in;
+    ed[ln] = D2[ln] - A2[ln] * D1[ln] * i1 - C2[ln] * d1n * in;
+  }
+  for (int st = 1; st < L; st <<= 1) {
+    double na[L], nb[L], nc[L], nd[L];
+    for (int ln = 0; ln < L; ln++) {
+      double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
+      if (ln >= st) { am = ea[ln - st]; bm = eb[ln - st]; cm = ec[ln - st]; dm = ed[ln - st]; }
+      if (ln + st < L) { ap = ea[ln + st]; bp = eb[ln + st]; cp = ec[ln + st]; dp = ed[ln + st]; }
+      double k1 = ea[ln] / bm, k2 = ec[ln] / bp;
+      na[ln] = -am * k1;
+      nc[ln] = -cp * k2;
+      nb[ln] = eb[ln] - cm * k1 - ap * k2;
+      nd[ln] = ed[ln] - dm * k1 - dp * k2;
+    }
+    memcpy(ea, na, sizeof(na));
+    memcpy(eb, nb, sizeof(nb));
+    memcpy(ec, nc, sizeof(nc));
+    memcpy(ed, nd, sizeof(nd));
+  }
+  for (int ln = 0; ln < L; ln++) xe[ln] = ed[ln] / eb[ln];
+  for (int ln = 0; ln < L; ln++) {
+    int s = sv[ln], m = mv[ln];
+    if (m < 2) continue;
+    double xp = ln == 0 ? 0.0 : xe[ln - 1];
+    xs[ln] = (D1[ln] - A1[ln] * xp - C1[ln] * xe[ln]) / B1[ln];
+    U[s] = xs[ln];
+    for (int i = 1; i < m - 1; i++) U[s + i] = d[s + i] - a[s + i] * xs[ln] - c[s + i] * xe[ln];
+    U[s + m - 1] = xe[ln];
+  }
+  free(w);
+  return 0;
+}
+
+static int g_line_solver = MTS_ORACLE_SOLVE_THOMAS;
+
+void mts_oracle_set_line_solver(int kind) { g_line_solver = kind; }
+
+int mts_oracle_line_solve(int n, const double *a, const double *b, const double *c,
+                          const double *r, double *u) {
+  if (n < 2) return -1;
+  if (g_line_solver == MTS_ORACLE_SOLVE_PIVOT) return gtsv(n, a, b, c, r, u);
+  if (g_line_solver == MTS_ORACLE_SOLVE_PARTITIONED) {
+    if (n < 4 || n > 1024) return -1;
+    return part_solve(n, a, b, c, r, u);
+  }
+  return mts_oracle_tdma(n, a, b, c, r, u);
+}
+
 /* Convert2Tri_irr (douglas_adi.cc:1554-1577) */
 static void convert_irr(int ix, double row[2][5], double *a, double *b, double *c, double *r) {
   double rate = row[1][0] / row[0][0];
@@ -471,7 +662,7 @@ static int sweep(Prob *P, int d, double *R, double coef, double dt, int jumps) {
           ip += 2;
         }
       }
-      if (mts_oracle_tdma(n, a, b, c, r, ut) != 0) return -1;
+      if (mts_oracle_line_solve(n, a, b, c, r, ut) != 0) return -1;
       for (int i = 0; i < n; i++) R[NODE(P, d, j, k, i)] = ut[i];
     }
   return 0;

@@ -47,6 +47,19 @@ void mts_oracle_norms(const mts_oracle_problem *p, const double *u, double t, do
 int mts_oracle_tdma(int n, const double *a, const double *b, const double *c, const double *r,
                     double *u);
 
+/* Line solver used by mts_oracle_run's implicit sweeps (process-wide):
+ *   THOMAS       the reference's TDMA (default),
+ *   PIVOT        tridiagonal Gaussian elimination with partial pivoting
+ *                (LAPACK dgtsv's algorithm): a conditioning probe,
+ *   PARTITIONED  a CPU model of the GPU's partitioned line solve (64 segments,
+ *                fold, cyclic reduction), operation for operation.
+ * Not part of the reference; tests use them to tell rounding from error. */
+enum { MTS_ORACLE_SOLVE_THOMAS = 0, MTS_ORACLE_SOLVE_PIVOT = 1, MTS_ORACLE_SOLVE_PARTITIONED = 2 };
+void mts_oracle_set_line_solver(int kind);
+/* One line with the selected solver. Returns -1 on a zero pivot or bad size. */
+int mts_oracle_line_solve(int n, const double *a, const double *b, const double *c,
+                          const double *r, double *u);
+
 #ifdef __cplusplus
 }
 #endif

@@ -48,6 +48,10 @@ def lib():
         L.mts_oracle_norms.restype = None
         L.mts_oracle_tdma.argtypes = [C.c_int] + [C.c_void_p] * 5
         L.mts_oracle_tdma.restype = C.c_int
+        L.mts_oracle_set_line_solver.argtypes = [C.c_int]
+        L.mts_oracle_set_line_solver.restype = None
+        L.mts_oracle_line_solve.argtypes = [C.c_int] + [C.c_void_p] * 5
+        L.mts_oracle_line_solve.restype = C.c_int
         _lib = L
     return _lib
 
@@ -132,6 +136,37 @@ def tdma(a, b, c, r):
     if lib().mts_oracle_tdma(len(r), a.ctypes.data, b.ctypes.data, c.ctypes.data, r.ctypes.data,
                              u.ctypes.data):
         raise ZeroDivisionError("zero pivot in tridiagonal solve")
+    return u
+
+
+SOLVERS = {"thomas": 0, "pivot": 1, "partitioned": 2}
+
+
+class line_solver:
+    """Context manager selecting the oracle's line solver ("thomas" is the
+    reference's TDMA; "pivot" partial-pivoting elimination; "partitioned" the
+    CPU model of the GPU's partitioned solve).  Restores Thomas on exit."""
+
+    def __init__(self, kind):
+        self.kind = SOLVERS[kind]
+
+    def __enter__(self):
+        lib().mts_oracle_set_line_solver(self.kind)
+        return self
+
+    def __exit__(self, *exc):
+        lib().mts_oracle_set_line_solver(0)
+        return False
+
+
+def line_solve(kind, a, b, c, r):
+    a, b, c, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c, r))
+    u = np.empty_like(r)
+    with line_solver(kind):
+        rc = lib().mts_oracle_line_solve(len(r), a.ctypes.data, b.ctypes.data, c.ctypes.data,
+                                         r.ctypes.data, u.ctypes.data)
+    if rc:
+        raise ZeroDivisionError("zero pivot or unsupported size in line solve")
     return u
 
 
