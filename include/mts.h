@@ -26,7 +26,8 @@ enum {
   MTS_E_GEOMETRY = -2,  /* setup refused: the reference would have exited (message says why) */
   MTS_E_DEVICE = -3,    /* HIP runtime error */
   MTS_E_STATE = -4,     /* call out of order */
-  MTS_E_NOMEM = -5
+  MTS_E_NOMEM = -5,
+  MTS_E_DISABLED = -6   /* combination switched off in this build (message says which and why) */
 };
 
 /* temporal methods (constant.h: Temporal_Method_Type) */

@@ -325,6 +325,15 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
   if (beta.kind != setup->beta.kind || beta.p[0] != setup->beta.p[0] ||
       beta.p[1] != setup->beta.p[1])
     return fail(MTS_E_ARG, "physics differs from the one the setup was built with");
+  if (method == MTS_ADI) {
+    const int *cp = setup->res.corner_pairs;
+    if (cp[0] + cp[1] + cp[2] > 0)
+      return fail(MTS_E_DISABLED,
+                  "Douglas-ADI on an interface with corner pairs is switched off on the GPU: "
+                  "its step raised a GPU memory aperture violation (dispatch reported as "
+                  "spec_kernel) whose cause is not found; Douglas-ADI without corner pairs, "
+                  "and LOD-IE, LOD-CN and TS with or without them, are not affected");
+  }
   MTS_HIP(hipSetDevice(ctx->device));
   std::unique_ptr<mts_instance> in(new mts_instance());
   in->ctx = ctx;

@@ -44,6 +44,25 @@ def test_gpu_matches_reference(mts, ctx, name):
     inst.close()
 
 
+DOUGLAS_CORNERS_OFF = (
+    "Douglas-ADI with corner pairs is switched off on the GPU: two runs (ellipsoid 65^3 "
+    "MIB-L1, torus 49^3 MIB-L2) raised a memory aperture violation (dispatch reported as "
+    "spec_kernel) whose cause is not found; the host bounds check of those cases is clean")
+
+
+def test_gpu_douglas_with_corners_is_refused(mts, ctx):
+    """The switched-off combination fails loudly at instance creation (no kernel runs)."""
+    phys = mts.physics("sincos", 10, 1)
+    st = mts.Setup(mts.grid(65), mts.surface("ellipsoid", 0.7, 0.5, 0.4, 1 - 2 ** -20), phys,
+                   mib=1)
+    assert sum(st.info().corner_pairs) > 0
+    with pytest.raises(mts.MtsError) as e:
+        mts.Instance(ctx, st, "adi", 1 / 32)
+    assert e.value.code == -6
+    inst = mts.Instance(ctx, st, "lod-ie", 1 / 32)  # other methods unaffected
+    inst.close()
+
+
 CASES = [
     # (surface, args, beta, n intervals, dt, steps)
     ("ellipsoid", (0.7, 0.5, 0.4, 1 - 2 ** -20), ("sincos", 10, 1), 64, 1 / 32, 3),
@@ -63,6 +82,8 @@ def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
     print(f"case {surf} n={n} {method} mib{mib}", flush=True)
     phys = mts.physics(*beta)
     st = mts.Setup(mts.grid(n + 1), mts.surface(surf, *sargs), phys, mib=mib)
+    if method == "adi" and sum(st.info().corner_pairs) > 0:
+        pytest.skip(DOUGLAS_CORNERS_OFF)
     op = oracle.OracleProblem.from_setup(st, phys.beta_kind, beta[1:], -1.0, 1.0)
     inst = mts.Instance(ctx, st, method, dt)
     inst.initialize(0.0)
@@ -118,7 +139,7 @@ def test_gpu_error_norms_match_oracle(mts, oracle, ctx):
 def test_gpu_layouts_round_trip(mts, ctx):
     phys = mts.physics("const", 10, 1)
     st = mts.Setup(mts.grid((17, 19, 21)), mts.surface("ellipsoid", 1, 1, 1, 0.47), phys, mib=1)
-    inst = mts.Instance(ctx, st, "adi", 0.1)
+    inst = mts.Instance(ctx, st, "lod-ie", 0.1)
     rng = np.random.default_rng(3)
     u = rng.standard_normal(inst.shape)
     inst.upload(u)
