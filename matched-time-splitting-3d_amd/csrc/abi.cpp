@@ -5,6 +5,7 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -12,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "device_core.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
 #include "kernels.hpp"
@@ -100,6 +102,7 @@ struct mts_ctx {
 struct mts_instance {
   mts_ctx *ctx = nullptr;
   int method = 0;
+  int min_lanes = 16;  // lane-group lower bound of the line split (tuning)
   double dt = 0.0, theta_dt = 0.0;
   mts::Grid grid;
   int px = 0;
@@ -113,6 +116,7 @@ struct mts_instance {
   DevBuf<double> blk_val[3];
   DevBuf<double> cr_val[3];
   DevBuf<uint8_t> seq[3];
+  DevBuf<uint8_t> cls0[3];
   DevBuf<int> spec;
   int nspec = 0;
   DevBuf<double> part_max, part_sq;
@@ -342,12 +346,17 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
   in->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
   in->grid = setup->res.grid;
   const mts::Grid &g = in->grid;
+  // MTS_LINE_LANES=32|64 raises the lane-group lower bound of the line split
+  // (a tuning knob: every split computes the same solution to rounding)
+  if (const char *ev = std::getenv("MTS_LINE_LANES")) {
+    const int v = std::atoi(ev);
+    if (v == 16 || v == 32 || v == 64) in->min_lanes = v;
+  }
   for (int d = 0; d < 3; d++) {
-    int n = g.n[d];
-    int p = n / 2 > 64 ? 64 : n / 2;
-    if ((n + p - 1) / p > 16) return fail(MTS_E_ARG, "line length above 1024 is not supported");
-    size_t tiles = (size_t)16 * (n + 1);
-    size_t need = tiles * sizeof(double) * 2 + tiles;
+    mts::LinePart q;
+    if (!mts::make_line_part(g.n[d], q, in->min_lanes))
+      return fail(MTS_E_ARG, "line length above 1024 nodes is not supported");
+    const size_t need = (size_t)mts::tile_lines(q.L) * q.lds_len * sizeof(double) * 2;
     if (need > ctx->prop.sharedMemPerBlock)
       return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
   }
@@ -357,7 +366,9 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
 
   mts::DeviceTableHost t;
   try {
-    mts::build_device_table(setup->res, beta, in->theta_dt, method == MTS_ADI, t);
+    const std::string terr =
+        mts::build_device_table(setup->res, beta, in->theta_dt, method == MTS_ADI, t);
+    if (!terr.empty()) return fail(MTS_E_GEOMETRY, terr);
   } catch (const std::bad_alloc &) {
     return fail(MTS_E_NOMEM, "out of host memory building the MIB table");
   }
@@ -388,6 +399,7 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
     MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size(), st));
     MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size(), st));
     if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
+    MTS_HIP(in->cls0[d].upload(t.cls0[d].data(), t.cls0[d].size(), st));
   }
   in->nspec = (int)(t.spec.size() / mts::kSpecLen);
   MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size(), st));
@@ -404,6 +416,7 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
     in->mib[d].val = in->blk_val[d].p;
     in->mib[d].cr = in->cr_val[d].p;
     in->mib[d].seq = in->seq[d].p;
+    in->mib[d].cls0 = in->cls0[d].p;
   }
   int nb = mts::norms_blocks(dg);
   MTS_HIP(in->part_max.alloc(nb));
@@ -481,6 +494,7 @@ int mts_initialize(mts_instance *in, double t0) {
 static mts::LineArgs base_args(mts_instance *in) {
   mts::LineArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.min_lanes = in->min_lanes;
   a.g = in->geom;
   a.P = in->phys;
   a.theta_dt = in->theta_dt;

@@ -38,66 +38,82 @@ struct Emu {
   PhysConst phys;
 };
 
-template <int DIR, int MMAX, int FLAGS>
+template <int DIR, int L, int MMAX, int FLAGS>
 void emulate_line(Emu &E, const LineArgs &p) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;
+  constexpr int LPW = 64 / L, T = tile_lines(L);
+  (void)E;
+  const LinePart &q = p.q;
   const int n = p.g.n[DIR];
-  const int ns = n + 1;
   const int nj = LineMap<DIR>::nj(p.g), nk = LineMap<DIR>::nk(p.g);
-  const int ntile = (nj + kTile - 1) / kTile;
+  const int ntile = (nj + T - 1) / T;
+  const unsigned magic_n = magic_of(n);
   for (int bx = 0; bx < ntile; bx++)
     for (int k = 1; k <= nk - 2; k++) {
       // exact-size LDS tiles
-      std::unique_ptr<double[]> tA(new double[kTile * ns]);
-      std::unique_ptr<double[]> tB(new double[kTile * ns]);
-      std::unique_ptr<int8_t[]> tI(new int8_t[kTile * ns]);
-      std::memset(tA.get(), 0, sizeof(double) * kTile * ns);
-      std::memset(tB.get(), 0, sizeof(double) * kTile * ns);
-      std::memset(tI.get(), 1, kTile * ns);
-      const int j0 = bx * kTile;
-      for (int e = 0; e < kTile * n; e++) tile_load<DIR, HAS_S>(p, j0, k, e, tA.get(), tB.get(), tI.get());
-      for (int l = 0; l < kTile; l++) {
-        const int j = j0 + l;
-        if (j < 1 || j > nj - 2) continue;
-        for (int lane = 0; lane < 64; lane++) {
-          int s, m;
-          segment(n, lane, s, m);
-          double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-          line_rows<DIR, MMAX, FLAGS>(p, j, k, tA.get() + l * ns, tB.get() + l * ns,
-                                      tI.get() + l * ns, s, m, a, b, c, d);
-          if ((FLAGS & kFlagSolve) && p.mib.seq) {  // the kernel's per-line solver choice
-            volatile uint8_t sq = p.mib.seq[k * nj + j];
-            (void)sq;
+      std::unique_ptr<double[]> tA(new double[(size_t)T * q.lds_len]);
+      std::unique_ptr<double[]> tB(new double[(size_t)T * q.lds_len]);
+      std::memset(tA.get(), 0, sizeof(double) * T * q.lds_len);
+      std::memset(tB.get(), 0, sizeof(double) * T * q.lds_len);
+      const int j0 = bx * T;
+      for (int e = 0; e < T * n; e++)
+        tile_load<DIR, HAS_S, T>(p, j0, k, e, magic_n, tA.get(), tB.get());
+      for (int wave = 0; wave < 4; wave++)
+          for (int lane = 0; lane < 64; lane++) {
+            const int kseg = lane & (L - 1);
+            const int l = wave * LPW + lane / L;
+            const int j = j0 + l;
+            const bool valid = j >= 1 && j <= nj - 2;
+            const LaneSeg sg = lane_seg(q, kseg, valid);
+            if (sg.m > 0 && (sg.m < 2 || sg.m > MMAX || sg.s < 1 || sg.s + sg.m > n - 1)) std::abort();
+            double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+            line_rows<DIR, MMAX, FLAGS>(p, j, k, tA.get() + l * q.lds_len,
+                                        tB.get() + l * q.lds_len, sg, a, b, c, d);
+            double x0 = 0.0, xn = 0.0;
+            if (valid) line_ends<DIR>(p, j, k, x0, xn);
+            if (SOLVE && valid && p.mib.seq) {  // the kernel's per-line solver choice
+              volatile uint8_t sq = p.mib.seq[k * nj + j];
+              (void)sq;
+            }
+            if (valid)
+              line_write<DIR, MMAX, FLAGS>(p, j, k, sg, d, x0, xn,
+                                           (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len);
           }
-          double xs = m > 0 ? d[0] : 0.0, xe = 0.0;
-          for (int q = 0; q < MMAX; q++)
-            if (q == m - 1) xe = d[q];
-          line_write<DIR, MMAX, FLAGS>(p, j, k, tI.get() + l * ns, s, m, d, xs, xe,
-                                       (TWO && !HAS_S ? tB.get() : tA.get()) + l * ns);
-        }
-      }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
-      for (int e = 0; e < kTile * n; e++) tile_store<DIR>(p, j0, k, e, tO);
+      for (int e = 0; e < T * n; e++) tile_store<DIR, T>(p, j0, k, e, magic_n, tO);
     }
 }
 
 template <int FLAGS>
-void line(Emu &E, int dir, const LineArgs &p) {
-  int n = p.g.n[dir];
-  int pl = n / 2 > 64 ? 64 : n / 2;
-  int mmax = (n + pl - 1) / pl;
-  auto run = [&](auto mm) {
-    constexpr int MM = decltype(mm)::value;
-    if (dir == 0) emulate_line<0, MM, FLAGS>(E, p);
-    else if (dir == 1) emulate_line<1, MM, FLAGS>(E, p);
-    else emulate_line<2, MM, FLAGS>(E, p);
+void line(Emu &E, int dir, const LineArgs &args) {
+  LineArgs p = args;
+  if (!make_line_part(p.g.n[dir], p.q, p.min_lanes)) std::abort();
+  auto run = [&](auto lc, auto mc) {
+    constexpr int LL = decltype(lc)::value;
+    constexpr int MM = decltype(mc)::value;
+    if (dir == 0) emulate_line<0, LL, MM, FLAGS>(E, p);
+    else if (dir == 1) emulate_line<1, LL, MM, FLAGS>(E, p);
+    else emulate_line<2, LL, MM, FLAGS>(E, p);
   };
-  if (mmax <= 4) run(std::integral_constant<int, 4>());
-  else if (mmax <= 8) run(std::integral_constant<int, 8>());
-  else run(std::integral_constant<int, 16>());
+  using I16 = std::integral_constant<int, 16>;
+  using I32 = std::integral_constant<int, 32>;
+  using I64 = std::integral_constant<int, 64>;
+  using M4 = std::integral_constant<int, 4>;
+  using M8 = std::integral_constant<int, 8>;
+  using M16 = std::integral_constant<int, 16>;
+  if (p.q.L == 16 && p.q.mmax == 4) run(I16(), M4());
+  else if (p.q.L == 16 && p.q.mmax == 8) run(I16(), M8());
+  else if (p.q.L == 16 && p.q.mmax == 16) run(I16(), M16());
+  else if (p.q.L == 32 && p.q.mmax == 4) run(I32(), M4());
+  else if (p.q.L == 32 && p.q.mmax == 8) run(I32(), M8());
+  else if (p.q.L == 32 && p.q.mmax == 16) run(I32(), M16());
+  else if (p.q.L == 64 && p.q.mmax == 4) run(I64(), M4());
+  else if (p.q.L == 64 && p.q.mmax == 8) run(I64(), M8());
+  else if (p.q.L == 64 && p.q.mmax == 16) run(I64(), M16());
+  else std::abort();
 }
 
 }  // namespace
@@ -109,7 +125,9 @@ int main(int argc, char **argv) {
   }
   std::string method = argv[1];
   int mib = std::atoi(argv[2]);
-  int nodes = std::atoi(argv[3]);
+  // nodes per axis: "N" or "NXxNYxNZ"
+  int nn[3];
+  if (std::sscanf(argv[3], "%dx%dx%d", &nn[0], &nn[1], &nn[2]) != 3) nn[0] = nn[1] = nn[2] = std::atoi(argv[3]);
   std::string sk = argv[4];
   int ai = 5;
   std::unique_ptr<Surface> surf;
@@ -130,7 +148,6 @@ int main(int argc, char **argv) {
   beta.p[0] = atof(argv[ai + 1]);
   beta.p[1] = atof(argv[ai + 2]);
   double lo[3] = {-1, -1, -1}, hi[3] = {1, 1, 1};
-  int nn[3] = {nodes, nodes, nodes};
   Grid g = make_grid(lo, hi, nn);
   SetupResult res;
   BuildOptions opt;
@@ -140,22 +157,26 @@ int main(int argc, char **argv) {
     std::printf("setup refused: %s\n", err.c_str());
     return 3;
   }
-  const double dt = 2.0 / (nodes - 1);
+  const double dt = 2.0 / (nn[0] - 1);
   const bool adi = method == "adi";
   const double tdt = (adi || method == "lod-ie") ? dt : 0.5 * dt;
   Emu E;
   E.s = &res;
-  build_device_table(res, beta, tdt, adi, E.t);
-  E.px = (nodes + 15) / 16 * 16;
-  E.fe = (size_t)E.px * nodes * nodes;
+  const std::string terr = build_device_table(res, beta, tdt, adi, E.t);
+  if (!terr.empty()) {
+    std::printf("table refused: %s\n", terr.c_str());
+    return 3;
+  }
+  E.px = (nn[0] + 15) / 16 * 16;
+  E.fe = (size_t)E.px * nn[1] * nn[2];
   E.u.assign(E.fe, 0.25);
   E.r.assign(E.fe, 0.0);
   E.ind.assign(E.fe, 1);
-  for (int iz = 0; iz < nodes; iz++)
-    for (int iy = 0; iy < nodes; iy++)
-      std::memcpy(&E.ind[((size_t)iz * nodes + iy) * E.px], &res.ind[g.idx(0, iy, iz)], nodes);
+  for (int iz = 0; iz < nn[2]; iz++)
+    for (int iy = 0; iy < nn[1]; iy++)
+      std::memcpy(&E.ind[((size_t)iz * nn[1] + iy) * E.px], &res.ind[g.idx(0, iy, iz)], nn[0]);
   for (int a = 0; a < 3; a++) {
-    E.g.n[a] = nodes;
+    E.g.n[a] = nn[a];
     E.g.tab[a] = E.t.tab[a].data();
   }
   E.g.px = E.px;
@@ -166,9 +187,12 @@ int main(int argc, char **argv) {
     E.mib[d].val = E.t.blk_val[d].data();
     E.mib[d].cr = E.t.cr_val[d].data();
     E.mib[d].seq = E.t.n_seq[d] > 0 ? E.t.seq[d].data() : nullptr;
+    E.mib[d].cls0 = E.t.cls0[d].data();
   }
   LineArgs a;
   std::memset(&a, 0, sizeof(a));
+  a.min_lanes = 16;
+  if (const char *ev = std::getenv("MTS_LINE_LANES")) a.min_lanes = std::atoi(ev);
   a.g = E.g;
   a.P = E.t.phys;
   a.theta_dt = tdt;
@@ -177,9 +201,9 @@ int main(int argc, char **argv) {
   a.c_f1 = a.c_f2 = 0.5 * dt;
   if (adi) {
     Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, 0.9};
-    for (int iz = 0; iz < nodes; iz++)
-      for (int iy = 0; iy < nodes; iy++)
-        for (int ix = 0; ix < nodes; ix++) rhs1_node(r1, ix, iy, iz);
+    for (int iz = 0; iz < nn[2]; iz++)
+      for (int iy = 0; iy < nn[1]; iy++)
+        for (int ix = 0; ix < nn[0]; ix++) rhs1_node(r1, ix, iy, iz);
     SpecArgs sp;
     sp.g = E.g;
     sp.P = E.t.phys;
@@ -231,7 +255,7 @@ int main(int argc, char **argv) {
   } else {
     return 2;
   }
-  std::printf("ok %s mib%d %d^3 crossings %zu %zu %zu\n", method.c_str(), mib, nodes,
+  std::printf("ok %s mib%d %dx%dx%d crossings %zu %zu %zu\n", method.c_str(), mib, nn[0], nn[1], nn[2],
               res.cross[0].size(), res.cross[1].size(), res.cross[2].size());
   return 0;
 }

@@ -14,6 +14,19 @@
 #else
 #define MTS_HD inline
 #endif
+// Keeps the instruction scheduler from hoisting the loads of later phases of
+// an MIB block above earlier ones (all 60 doubles of a corner record live at
+// once would cap the kernel's occupancy).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MTS_PHASE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define MTS_PHASE() ((void)0)
+#endif
+#if defined(__HIPCC__)
+#define MTS_OUTLINE __host__ __device__ __attribute__((noinline))
+#else
+#define MTS_OUTLINE inline
+#endif
 
 namespace mts {
 
@@ -74,9 +87,9 @@ MTS_HD void op_weights(const PhysConst &P, const DevGeom &g, int d,
     b1 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabCos, iy));
     b2 = b1;
   }
-  vd[0] = b1 / P.hh[d];
-  vd[1] = -(b1 + b2) / P.hh[d];
-  vd[2] = b2 / P.hh[d];
+  vd[0] = b1 * P.ihh[d];
+  vd[1] = -(b1 + b2) * P.ihh[d];
+  vd[2] = b2 * P.ihh[d];
 }
 
 // [u] and u_dir = P^{-1}[d][0] n.[beta grad u] at one crossing
@@ -263,7 +276,76 @@ MTS_HD void spec_node(const SpecArgs &p, int t) {
 }
 
 // ------------------------------------------------------- line stages
-constexpr int kTile = 16;  // lines per workgroup tile
+// A workgroup of 256 threads solves one tile of lines at once: 256 / L
+// lines (16 lines of a 16-lane split, 8 of a 32-lane one, 4 of a 64-lane one).
+constexpr int kBlockThreads = 256;
+MTS_HD constexpr int tile_lines(int L) { return kBlockThreads / L; }
+
+// Split of an n-node line over a lane group (see LinePart).  Picks the
+// smallest group (16, then 32, then 64 lanes, at least min_lanes) and
+// register bound (4, 8, 16 rows per lane) that hold the interior rows with no
+// segment of one row.
+inline bool make_line_part(int n, LinePart &q, int min_lanes = 16) {
+  const int ni = n - 2;
+  if (ni < 2 || n > 1024) return false;
+  static const int kL[3] = {16, 32, 64};
+  static const int kM[3] = {4, 8, 16};
+  for (int li = 0; li < 3; li++) {
+    const int L = kL[li];
+    if (L < min_lanes) continue;
+    int m = (ni + L - 1) / L;
+    if (m < 2) m = 2;
+    int p = (ni + m - 1) / m;
+    int mlast = ni - (p - 1) * m;
+    if (mlast == 1) {  // fold a one-row tail into the previous segment
+      p -= 1;
+      mlast = m + 1;
+    }
+    if (p > L) continue;
+    const int mx = m > mlast ? m : mlast;
+    for (int mi = 0; mi < 3; mi++) {
+      if (mx > kM[mi]) continue;
+      q.n = n;
+      q.L = L;
+      q.mmax = kM[mi];
+      q.m = m;
+      q.mlast = mlast;
+      q.p = p;
+      q.stride = mx | 1;
+      q.magic = ((1 << 20) + m - 1) / m;
+      q.lds_len = 2 + p * q.stride;
+      return true;
+    }
+  }
+  return false;
+}
+
+// LDS slot of row r of a line
+MTS_HD int lds_slot(const LinePart &q, int r) {
+  if (r <= 0) return 0;
+  if (r >= q.n - 1) return 1 + q.p * q.stride;
+  const int t = r - 1;
+  int k = (t * q.magic) >> 20;
+  if (k > q.p - 1) k = q.p - 1;
+  return 1 + k * q.stride + (t - k * q.m);
+}
+
+// Segment of lane kseg of a group: first row s, m rows (0: idle lane).
+struct LaneSeg {
+  int k, s, m;
+};
+MTS_HD LaneSeg lane_seg(const LinePart &q, int kseg, bool line_valid) {
+  LaneSeg r;
+  r.k = kseg;
+  if (!line_valid || kseg >= q.p) {
+    r.s = q.n - 1;
+    r.m = 0;
+  } else {
+    r.s = 1 + kseg * q.m;
+    r.m = kseg == q.p - 1 ? q.mlast : q.m;
+  }
+  return r;
+}
 
 template <int DIR>
 struct LineMap {
@@ -284,29 +366,33 @@ struct LineMap {
   }
 };
 
-// tile element e -> (line l, row r): x lines are contiguous rows, y/z lines
-// put 16 consecutive lines (one 128-byte segment) next to each other
-template <int DIR>
-MTS_HD void tile_elem(int e, int n, int &l, int &r) {
+// Tile element e -> (line l, row r) for a tile of T lines.  y/z lines: T
+// consecutive lines (one segment of an x row) per line row; x lines: one
+// contiguous line after the other (l = e / n by a multiply-high with
+// magic_n = ceil(2^32 / n)).
+template <int DIR, int T>
+MTS_HD void tile_elem(int e, int n, unsigned magic_n, int &l, int &r) {
   if (DIR == 0) {
-    l = e / n;
+    l = (int)(((unsigned long long)(unsigned)e * magic_n) >> 32);
     r = e - l * n;
   } else {
-    l = e & (kTile - 1);
-    r = e / kTile;
+    l = e & (T - 1);
+    r = e / T;
   }
 }
+MTS_HD unsigned magic_of(int n) {
+  return (unsigned)((0x100000000ull + (unsigned long long)n - 1) / (unsigned long long)n);
+}
 
-// Stage one element of the tile's input field(s) and classification in LDS.
-template <int DIR, bool HAS_S>
-MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, double *tA, double *tB,
-                      int8_t *tI) {
+// Stage one element of the tile's input field(s) in LDS.
+template <int DIR, bool HAS_S, int T>
+MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, unsigned magic_n, double *tA,
+                      double *tB) {
   using M = LineMap<DIR>;
   const DevGeom &g = p.g;
   const int n = g.n[DIR];
-  const int ns = n + 1;
   int l, r;
-  tile_elem<DIR>(e, n, l, r);
+  tile_elem<DIR, T>(e, n, magic_n, l, r);
   const int j = j0 + l;
   if (j >= M::nj(g)) return;
   const size_t gi = M::base(g, j, k) + (size_t)r * M::stride(g);
@@ -316,215 +402,334 @@ MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, double *tA, doubl
     M::node(j, k, r, ix, iy, iz);
     va = outer_u(g, p.e_in, ix, iy, iz);
   }
-  tA[l * ns + r] = va;
-  if (HAS_S) tB[l * ns + r] = p.S[gi];
-  tI[l * ns + r] = g.ind[gi];
+  const int sl = l * p.q.lds_len + lds_slot(p.q, r);
+  tA[sl] = va;
+  if (HAS_S) tB[sl] = p.S[gi];
 }
 
 // Write one element of the tile's result back (interior lines only).
-template <int DIR>
-MTS_HD void tile_store(const LineArgs &p, int j0, int k, int e, const double *tO) {
+template <int DIR, int T>
+MTS_HD void tile_store(const LineArgs &p, int j0, int k, int e, unsigned magic_n,
+                       const double *tO) {
   using M = LineMap<DIR>;
   const DevGeom &g = p.g;
   const int n = g.n[DIR];
   int l, r;
-  tile_elem<DIR>(e, n, l, r);
+  tile_elem<DIR, T>(e, n, magic_n, l, r);
   const int j = j0 + l;
   if (j < 1 || j > M::nj(g) - 2) return;
-  p.out[M::base(g, j, k) + (size_t)r * M::stride(g)] = tO[l * (n + 1) + r];
+  p.out[M::base(g, j, k) + (size_t)r * M::stride(g)] = tO[l * p.q.lds_len + lds_slot(p.q, r)];
+}
+
+// Classification (true = outside) of row r of line lid: the line's first
+// node class flipped at every crossing left of r (an irregular block holds
+// one crossing at ll, a corner pair two, at ll and ll + 1).
+MTS_HD bool row_outside(const DevMib &mib, int lid, int r) {
+  bool o = mib.cls0[lid] != 0;
+  const int bb = mib.blk_ptr[lid], be = mib.blk_ptr[lid + 1];
+  for (int bi = bb; bi < be; bi++) {
+    const BlockHdr h = mib.hdr[bi];
+    if (h.ll < r) o = !o;
+    if (h.type == kBlkCor && h.ll + 1 < r) o = !o;
+  }
+  return o;
+}
+// Classification bits of rows s .. s + 31 (bit t set: row s + t outside).
+MTS_HD unsigned rows_outside(const DevMib &mib, int lid, int s) {
+  unsigned mask = mib.cls0[lid] ? 0xffffffffu : 0u;
+  const int bb = mib.blk_ptr[lid], be = mib.blk_ptr[lid + 1];
+  for (int bi = bb; bi < be; bi++) {
+    const BlockHdr h = mib.hdr[bi];
+    const int nf = h.type == kBlkCor ? 2 : 1;
+    for (int f = 0; f < nf; f++) {
+      const int t0 = h.ll + f - s + 1;  // rows s + t with t >= t0 lie right of the crossing
+      if (t0 <= 0) mask = ~mask;
+      else if (t0 < 32) mask ^= ~((1u << t0) - 1u);
+    }
+  }
+  return mask;
+}
+
+// Right-hand side of the patched rows of one MIB block (douglas_adi.cc:116-195
+// and :405-482, lod.cc:414-440 and :887-1078): fictitious-point stencil sums,
+// jump terms and the Convert2Tri row reduction.  The block's rows ll-1 ..
+// ll+3 of the input field(s) are read from LDS once (row indices past the
+// line's end read its last row, unused).
+struct BlockRhs {
+  double v0, v1, v2;
+};
+template <int DIR, int FLAGS>
+MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, int lid, const BlockHdr h,
+                          const double *la, const double *lb) {
+  constexpr int RHS = FLAGS & 3;
+  constexpr bool LJ = (FLAGS & kFlagLhsJump) != 0;
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const LinePart &q = p.q;
+  const double tdt = p.theta_dt;
+  const int ll = h.ll;
+  double fa[5], fs[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const int sl = lds_slot(q, ll - 1 + t);
+    fa[t] = la[sl];
+    fs[t] = RHS == kRhsDouglas ? lb[sl] : 0.0;
+  }
+  auto FA = [&](int i) { return fa[i - (ll - 1)]; };
+  auto FS = [&](int i) { return fs[i - (ll - 1)]; };
+  const double *v = p.mib.val + h.off;
+  double ju = 0.0, jb = 0.0, juR = 0.0, jbR = 0.0;
+  if (RHS == kRhsCN || RHS == kRhsTSZ || LJ) {
+    jumps(p.mib.cr + (size_t)h.c0 * kCrLen, p.e_jump, ju, jb);
+    if (h.type == kBlkCor) jumps(p.mib.cr + (size_t)(h.c0 + 1) * kCrLen, p.e_jump, juR, jbR);
+  }
+  double bv[3] = {0.0, 0.0, 0.0};
+  if (h.type == kBlkIrr) {
+    const double *vd0 = v + kIrrVd, *vd1 = v + kIrrVd + 3;
+    const double *wl = v + kIrrWl, *wr = v + kIrrWr;
+    const double J0 = ju * wr[4] + jb * wr[5];
+    const double J1 = ju * wl[4] + jb * wl[5];
+    if (RHS == kRhsCopy) {
+      bv[0] = FA(ll);
+      bv[1] = FA(ll + 1);
+    } else if (RHS == kRhsDouglas) {
+      bv[0] = -p.dt * irr_sum0(v, FS, ll) + FA(ll);
+      bv[1] = -p.dt * irr_sum1(v, FS, ll) + FA(ll + 1);
+    } else {
+      double s0 = irr_sum0(v, FA, ll) + vd0[2] * J0;
+      MTS_PHASE();
+      double s1 = irr_sum1(v, FA, ll) + vd1[0] * J1;
+      MTS_PHASE();
+      bv[0] = tdt * s0 + FA(ll);
+      bv[1] = tdt * s1 + FA(ll + 1);
+    }
+    if (RHS == kRhsTSZ) {
+      for (int r = 0; r < 2; r++) {
+        int ix, iy, iz;
+        M::node(j, k, ll + r, ix, iy, iz);
+        const bool out = row_outside(p.mib, lid, ll + r);
+        bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+        bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+      }
+    }
+    if (LJ) {
+      bv[0] = bv[0] + tdt * vd0[2] * J0;
+      bv[1] = bv[1] + tdt * vd1[0] * J1;
+    }
+    if (SOLVE) {  // Convert2Tri row reduction, only where the rows are solved
+      const double *mu = v + kIrrMul;
+      bv[1] -= mu[0] * bv[0];
+      bv[0] -= mu[1] * bv[1];
+    }
+  } else {
+    const double *vd0 = v + kCorVd, *vd1 = v + kCorVd + 3, *vd2 = v + kCorVd + 6;
+    const double JLwr = jl(v + kCorLwr, ju, jb, juR, jbR);
+    const double JLwl = jl(v + kCorLwl, ju, jb, juR, jbR);
+    const double JRwr = jl(v + kCorRwr, ju, jb, juR, jbR);
+    const double JRwl = jl(v + kCorRwl, ju, jb, juR, jbR);
+    MTS_PHASE();
+    if (RHS == kRhsCopy) {
+      bv[0] = FA(ll);
+      bv[1] = FA(ll + 1);
+      bv[2] = FA(ll + 2);
+    } else if (RHS == kRhsDouglas) {
+      bv[0] = -p.dt * cor_sum0(v, FS, ll) + FA(ll);
+      MTS_PHASE();
+      bv[1] = -p.dt * cor_sum1(v, FS, ll, false, 0.0, 0.0) + FA(ll + 1);
+      MTS_PHASE();
+      bv[2] = -p.dt * cor_sum2(v, FS, ll) + FA(ll + 2);
+      MTS_PHASE();
+    } else {
+      double s0 = cor_sum0(v, FA, ll) + vd0[2] * JLwr;
+      MTS_PHASE();
+      double s1 = cor_sum1(v, FA, ll, true, JLwl, JRwr);
+      MTS_PHASE();
+      double s2 = cor_sum2(v, FA, ll) + vd2[0] * JRwl;
+      MTS_PHASE();
+      bv[0] = tdt * s0 + FA(ll);
+      bv[1] = tdt * s1 + FA(ll + 1);
+      bv[2] = tdt * s2 + FA(ll + 2);
+    }
+    if (RHS == kRhsTSZ) {
+      for (int r = 0; r < 3; r++) {
+        int ix, iy, iz;
+        M::node(j, k, ll + r, ix, iy, iz);
+        const bool out = row_outside(p.mib, lid, ll + r);
+        bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+        bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+      }
+    }
+    if (LJ) {
+      bv[0] = bv[0] + tdt * vd0[2] * JLwr;
+      bv[1] = bv[1] + tdt * vd1[0] * JLwl;
+      bv[1] += tdt * vd1[2] * JRwr;
+      bv[2] = bv[2] + tdt * vd2[0] * JRwl;
+    }
+    if (SOLVE) {
+      const double *mu = v + kCorMul;
+      bv[1] -= mu[0] * bv[0];
+      bv[2] -= mu[1] * bv[0];
+      bv[0] -= mu[2] * bv[2];
+      bv[1] -= mu[3] * bv[2];
+      bv[2] -= mu[4] * bv[1];
+      bv[0] -= mu[5] * bv[1];
+    }
+  }
+  BlockRhs r;
+  r.v0 = bv[0];
+  r.v1 = bv[1];
+  r.v2 = bv[2];
+  return r;
 }
 
 // Rows [s, s + m) of line (j, k): tridiagonal coefficients and right-hand
 // side of the stage, including the line's MIB blocks (the reference's
-// per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).
-// la/lb/li: the line's input field(s) and classification staged in LDS.
+// per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
+// couplings of rows 1 and n-2 to the boundary rows stay in a and c.
+// la/lb: the line's input field(s) staged in LDS (slot layout of LinePart).
 template <int DIR, int MMAX, int FLAGS>
 MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const double *lb,
-                      const int8_t *li, int s, int m, double (&a)[MMAX], double (&b)[MMAX],
+                      const LaneSeg &sg, double (&a)[MMAX], double (&b)[MMAX],
                       double (&c)[MMAX], double (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool LJ = (FLAGS & kFlagLhsJump) != 0;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   using M = LineMap<DIR>;
   const DevGeom &g = p.g;
-  const int n = g.n[DIR];
+  const LinePart &q = p.q;
   const int nj = M::nj(g);
   const double tdt = p.theta_dt;
   const double mtdt = -tdt;
-  auto FA = [&](int i) { return la[i]; };
-  auto FS = [&](int i) { return lb[i]; };
+  const int s = sg.s, m = sg.m;
 #pragma unroll
-    for (int r = 0; r < MMAX; r++) {
-      a[r] = 0.0;
-      b[r] = 1.0;
-      c[r] = 0.0;
-      d[r] = 0.0;
-      if (r < m) {
-        const int i = s + r;
-        int ix, iy, iz;
-        M::node(j, k, i, ix, iy, iz);
-        if (i == 0 || i == n - 1) {
-          d[r] = outer_u(g, p.e_out, ix, iy, iz);
-        } else {
-          const bool out = li[i] > 0;
-          double vd[3];
-          op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
-          a[r] = mtdt * vd[0];
-          b[r] = 1 - tdt * vd[1];
-          c[r] = mtdt * vd[2];
-          if (RHS == kRhsCopy) {
-            d[r] = la[i];
-          } else if (RHS == kRhsDouglas) {
-            double sum = lb[i - 1] * vd[0] + lb[i] * vd[1] + lb[i + 1] * vd[2];
-            d[r] = -p.dt * sum + la[i];
-          } else {
-            double sum = la[i - 1] * vd[0] + la[i] * vd[1] + la[i + 1] * vd[2];
-            double v = tdt * sum + la[i];
-            if (RHS == kRhsTSZ) {
-              v = v + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
-              v = v + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
-            }
-            d[r] = v;
-          }
-        }
-      }
-    }
-
-    // ---- MIB blocks of this line
-    const int lid = k * nj + j;
-    const int bb = p.mib.blk_ptr[lid], be = p.mib.blk_ptr[lid + 1];
-    for (int bi = bb; bi < be; bi++) {
-      const BlockHdr h = p.mib.hdr[bi];
-      const int ll = h.ll;
-      const int nr = (h.type == kBlkIrr) ? 2 : 3;
-      if (m == 0 || ll + nr - 1 < s || ll >= s + m) continue;
-      const double *v = p.mib.val + h.off;
-      double ju = 0.0, jb = 0.0, juR = 0.0, jbR = 0.0;
-      if (RHS == kRhsCN || RHS == kRhsTSZ || LJ) {
-        jumps(p.mib.cr + (size_t)h.c0 * kCrLen, p.e_jump, ju, jb);
-        if (h.type == kBlkCor) jumps(p.mib.cr + (size_t)(h.c0 + 1) * kCrLen, p.e_jump, juR, jbR);
-      }
-      double bv[3];
-      const double *lhs;
-      if (h.type == kBlkIrr) {
-        const double *vd0 = v + kIrrVd, *vd1 = v + kIrrVd + 3;
-        const double *wl = v + kIrrWl, *wr = v + kIrrWr;
-        const double J0 = ju * wr[4] + jb * wr[5];
-        const double J1 = ju * wl[4] + jb * wl[5];
-        if (RHS == kRhsCopy) {
-          bv[0] = la[ll];
-          bv[1] = la[ll + 1];
-        } else if (RHS == kRhsDouglas) {
-          bv[0] = -p.dt * irr_sum0(v, FS, ll) + la[ll];
-          bv[1] = -p.dt * irr_sum1(v, FS, ll) + la[ll + 1];
-        } else {
-          double s0 = irr_sum0(v, FA, ll) + vd0[2] * J0;
-          double s1 = irr_sum1(v, FA, ll) + vd1[0] * J1;
-          bv[0] = tdt * s0 + la[ll];
-          bv[1] = tdt * s1 + la[ll + 1];
-        }
-        if (RHS == kRhsTSZ) {
-          for (int r = 0; r < 2; r++) {
-            int ix, iy, iz;
-            M::node(j, k, ll + r, ix, iy, iz);
-            const bool out = li[ll + r] > 0;
-            bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
-            bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
-          }
-        }
-        if (LJ) {
-          bv[0] = bv[0] + tdt * vd0[2] * J0;
-          bv[1] = bv[1] + tdt * vd1[0] * J1;
-        }
-        if (SOLVE) {  // Convert2Tri row reduction, only where the rows are solved
-          const double *mu = v + kIrrMul;
-          bv[1] -= mu[0] * bv[0];
-          bv[0] -= mu[1] * bv[1];
-        }
-        lhs = v + kIrrLhs;
-      } else {
-        const double *vd0 = v + kCorVd, *vd1 = v + kCorVd + 3, *vd2 = v + kCorVd + 6;
-        const double JLwr = jl(v + kCorLwr, ju, jb, juR, jbR);
-        const double JLwl = jl(v + kCorLwl, ju, jb, juR, jbR);
-        const double JRwr = jl(v + kCorRwr, ju, jb, juR, jbR);
-        const double JRwl = jl(v + kCorRwl, ju, jb, juR, jbR);
-        if (RHS == kRhsCopy) {
-          bv[0] = la[ll];
-          bv[1] = la[ll + 1];
-          bv[2] = la[ll + 2];
-        } else if (RHS == kRhsDouglas) {
-          bv[0] = -p.dt * cor_sum0(v, FS, ll) + la[ll];
-          bv[1] = -p.dt * cor_sum1(v, FS, ll, false, 0.0, 0.0) + la[ll + 1];
-          bv[2] = -p.dt * cor_sum2(v, FS, ll) + la[ll + 2];
-        } else {
-          double s0 = cor_sum0(v, FA, ll) + vd0[2] * JLwr;
-          double s1 = cor_sum1(v, FA, ll, true, JLwl, JRwr);
-          double s2 = cor_sum2(v, FA, ll) + vd2[0] * JRwl;
-          bv[0] = tdt * s0 + la[ll];
-          bv[1] = tdt * s1 + la[ll + 1];
-          bv[2] = tdt * s2 + la[ll + 2];
-        }
-        if (RHS == kRhsTSZ) {
-          for (int r = 0; r < 3; r++) {
-            int ix, iy, iz;
-            M::node(j, k, ll + r, ix, iy, iz);
-            const bool out = li[ll + r] > 0;
-            bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
-            bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
-          }
-        }
-        if (LJ) {
-          bv[0] = bv[0] + tdt * vd0[2] * JLwr;
-          bv[1] = bv[1] + tdt * vd1[0] * JLwl;
-          bv[1] += tdt * vd1[2] * JRwr;
-          bv[2] = bv[2] + tdt * vd2[0] * JRwl;
-        }
-        if (SOLVE) {
-          const double *mu = v + kCorMul;
-          bv[1] -= mu[0] * bv[0];
-          bv[2] -= mu[1] * bv[0];
-          bv[0] -= mu[2] * bv[2];
-          bv[1] -= mu[3] * bv[2];
-          bv[2] -= mu[4] * bv[1];
-          bv[0] -= mu[5] * bv[1];
-        }
-        lhs = v + kCorLhs;
-      }
+  for (int r = 0; r < MMAX; r++) {
+    a[r] = 0.0;
+    b[r] = 1.0;
+    c[r] = 0.0;
+    d[r] = 0.0;
+  }
+  if (m == 0) return;
+  const int lid = k * nj + j;
+  const unsigned cm = rows_outside(p.mib, lid, s);
+  const int base = 1 + sg.k * q.stride;
+  // the rows just before and after the segment
+  const int sl_prev = sg.k == 0 ? 0 : base - q.stride + q.m - 1;
+  const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
+  auto FA = [&](int i) { return la[lds_slot(q, i)]; };
+  auto FS = [&](int i) { return lb[lds_slot(q, i)]; };
 #pragma unroll
-      for (int r = 0; r < MMAX; r++) {
-        const int rr = s + r - ll;
-        if (r < m && rr >= 0 && rr < nr) {
-          if (SOLVE) {
-            a[r] = lhs[rr * 3 + 0];
-            b[r] = lhs[rr * 3 + 1];
-            c[r] = lhs[rr * 3 + 2];
-          }
-          d[r] = rr == 0 ? bv[0] : rr == 1 ? bv[1] : bv[2];
-        }
-      }
-    }
-
-}
-
-// Epilogue of lane rows and write to the LDS output tile.
-template <int DIR, int MMAX, int FLAGS>
-MTS_HD void line_write(const LineArgs &p, int j, int k, const int8_t *li, int s, int m,
-                       const double (&d)[MMAX], double xs, double xe, double *lo) {
-  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
-  constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
-  using M = LineMap<DIR>;
   for (int r = 0; r < MMAX; r++) {
     if (r < m) {
       const int i = s + r;
-      double x = SOLVE ? (r == 0 ? xs : (r == m - 1 ? xe : d[r])) : d[r];
-      if (EPI) {
-        int ix, iy, iz;
-        M::node(j, k, i, ix, iy, iz);
-        x = x + p.c_f1 * source(p.P, p.g, p.e_f1, li[i] > 0, ix, iy, iz);
+      int ix, iy, iz;
+      M::node(j, k, i, ix, iy, iz);
+      const bool out = ((cm >> r) & 1u) != 0;
+      double vd[3];
+      op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
+      a[r] = mtdt * vd[0];
+      b[r] = 1 - tdt * vd[1];
+      c[r] = mtdt * vd[2];
+      if (RHS == kRhsCopy) {
+        d[r] = la[base + r];
+      } else {
+        const double *f = RHS == kRhsDouglas ? lb : la;
+        const double fp = r == 0 ? f[sl_prev] : f[base + r - 1];
+        const double fc = f[base + r];
+        const double fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
+        const double sum = fp * vd[0] + fc * vd[1] + fn * vd[2];
+        if (RHS == kRhsDouglas) {
+          d[r] = -p.dt * sum + la[base + r];
+        } else {
+          double v = tdt * sum + fc;
+          if (RHS == kRhsTSZ) {
+            v = v + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
+            v = v + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+          }
+          d[r] = v;
+        }
       }
-      lo[i] = x;
     }
   }
+
+  // ---- MIB blocks of this line
+  const int bb = p.mib.blk_ptr[lid], be = p.mib.blk_ptr[lid + 1];
+  for (int bi = bb; bi < be; bi++) {
+    const BlockHdr h = p.mib.hdr[bi];
+    const int ll = h.ll;
+    const int nr = (h.type == kBlkIrr) ? 2 : 3;
+    if (ll + nr - 1 < s || ll >= s + m) continue;
+    const BlockRhs br = block_rhs<DIR, FLAGS>(p, j, k, lid, h, la, lb);
+    const double *lhs = p.mib.val + h.off + (h.type == kBlkIrr ? kIrrLhs : kCorLhs);
+    double bv[3] = {br.v0, br.v1, br.v2};
+    // The right-hand side goes in as a select of register values; a select
+    // between a global load and an array element would be turned into a load
+    // through a selected pointer, which keeps the array in scratch memory.
+    const double bv0 = bv[0], bv1 = bv[1], bv2 = bv[2];
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      const int rr = s + r - ll;
+      const bool hit = r < m && rr >= 0 && rr < nr;
+      if (SOLVE && hit) {
+        a[r] = lhs[rr * 3 + 0];
+        b[r] = lhs[rr * 3 + 1];
+        c[r] = lhs[rr * 3 + 2];
+      }
+      const double dv = rr == 0 ? bv0 : (rr == 1 ? bv1 : bv2);
+      d[r] = hit ? dv : d[r];
+    }
+  }
+}
+
+// Epilogue of the lane's rows into the LDS output tile; the group's lane 0
+// also writes the two boundary rows (x0, xn: their values).
+template <int DIR, int MMAX, int FLAGS>
+MTS_HD void line_write(const LineArgs &p, int j, int k, const LaneSeg &sg,
+                       const double (&x)[MMAX], double x0, double xn, double *lo) {
+  constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
+  using M = LineMap<DIR>;
+  const LinePart &q = p.q;
+  const int nj = M::nj(p.g);
+  const int lid = k * nj + j;
+  if (sg.m > 0) {
+    const unsigned cm = EPI ? rows_outside(p.mib, lid, sg.s) : 0u;
+    const int base = 1 + sg.k * q.stride;
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      if (r < sg.m) {
+        double v = x[r];
+        if (EPI) {
+          int ix, iy, iz;
+          M::node(j, k, sg.s + r, ix, iy, iz);
+          v = v + p.c_f1 * source(p.P, p.g, p.e_f1, ((cm >> r) & 1u) != 0, ix, iy, iz);
+        }
+        lo[base + r] = v;
+      }
+    }
+  }
+  if (sg.k == 0 && sg.m > 0) {
+    const int n = q.n;
+    if (EPI) {
+      int ix, iy, iz;
+      M::node(j, k, 0, ix, iy, iz);
+      x0 = x0 + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lid, 0), ix, iy, iz);
+      M::node(j, k, n - 1, ix, iy, iz);
+      xn = xn + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lid, n - 1), ix, iy, iz);
+    }
+    lo[0] = x0;
+    lo[1 + q.p * q.stride] = xn;
+  }
+}
+
+// Boundary values of line (j, k) for the output time (rows 0 and n-1).
+template <int DIR>
+MTS_HD void line_ends(const LineArgs &p, int j, int k, double &x0, double &xn) {
+  using M = LineMap<DIR>;
+  int ix, iy, iz;
+  M::node(j, k, 0, ix, iy, iz);
+  x0 = outer_u(p.g, p.e_out, ix, iy, iz);
+  M::node(j, k, p.q.n - 1, ix, iy, iz);
+  xn = outer_u(p.g, p.e_out, ix, iy, iz);
 }
 
 }  // namespace mts

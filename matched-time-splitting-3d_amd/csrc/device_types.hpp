@@ -52,6 +52,7 @@ struct PhysConst {
   double b_in, b_out;     // beta- (or its amplitude c) and beta+
   double f_in, f_out;     // 3*beta - 1 for the constant sides
   double hh[3];           // h*h per axis
+  double ihh[3];          // 1 / (h*h) per axis
   double vdc[3][2][3];    // operator weights of a constant-beta side [axis][0 in,1 out][3]
 };
 
@@ -71,12 +72,37 @@ struct DevGeom {
 // solved with the sequential Thomas algorithm (the reference's TDMA order):
 // the partitioned solve is stable only for diagonally dominant lines.  Null
 // when no line of the direction needs it.
+// cls0: one byte per line, 1 when the line's first node is outside the
+// interface.  Along a line the classification flips at every crossing (the
+// crossing scan records every sign change), so a kernel recovers the class of
+// any row from cls0 and the line's blocks without reading the per-node
+// classification.
 struct DevMib {
   const int *blk_ptr;
   const BlockHdr *hdr;
   const double *val;
   const double *cr;
   const uint8_t *seq;
+  const uint8_t *cls0;
+};
+
+// How a line of n nodes is split over the lanes of one lane group.  Rows 0
+// and n-1 (boundary values) are kept out of the split; interior rows
+// 1..n-2 form p segments of m rows (the last one mlast rows, 2 <= mlast),
+// one per lane of a group of L lanes (lanes p..L-1 idle).  In LDS a line
+// occupies lds_len double slots: row 0 at slot 0, segment k at
+// 1 + k * stride (stride odd, so that the lanes of a group start on distinct
+// banks), row n-1 at slot 1 + p * stride.
+struct LinePart {
+  int n;
+  int L;       // lanes per line: 16, 32 or 64
+  int mmax;    // register rows per lane (compile-time bound of the kernel)
+  int m;       // rows per segment except the last
+  int mlast;   // rows of the last segment
+  int p;       // segments
+  int stride;  // LDS slots per segment
+  int magic;   // ceil(2^20 / m): t / m == (t * magic) >> 20 for t < 1024
+  int lds_len; // LDS slots per line
 };
 
 // Right-hand-side kinds of a line stage.
@@ -94,6 +120,8 @@ enum : int {
 struct LineArgs {
   DevGeom g;
   DevMib mib;
+  LinePart q;          // filled at launch from the line length
+  int min_lanes;       // lane-group lower bound for the split (16: the default)
   PhysConst P;
   const double *A;     // stage input
   const double *S;     // second input (Douglas y/z: u^n), may be null

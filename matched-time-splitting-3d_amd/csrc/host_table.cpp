@@ -52,8 +52,8 @@ namespace {
 bool dominant(double a, double b, double c) { return !(std::fabs(b) < std::fabs(a) + std::fabs(c)); }
 }  // namespace
 
-void build_device_table(const SetupResult &s, const BetaModel &beta, double theta_dt,
-                        bool with_special, DeviceTableHost &t) {
+std::string build_device_table(const SetupResult &s, const BetaModel &beta, double theta_dt,
+                               bool with_special, DeviceTableHost &t) {
   const Grid &g = s.grid;
   const double mt = -theta_dt;
   std::unordered_map<size_t, int> spec_of;
@@ -221,6 +221,29 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
     }
     t.blk_ptr[d][nl] = (int)t.blk_hdr[d].size();
     for (int lid = 0; lid < nl; lid++) t.n_seq[d] += t.seq[d][lid];
+    // first-node class per line, and the check that the classification of
+    // every node is that class flipped at each crossing to its left (the
+    // kernels recover it that way)
+    t.cls0[d].assign((size_t)nl, 0);
+    const int n = g.n[d];
+    for (int lid = 0; lid < nl; lid++) {
+      const int j = lid % nj, k = lid / nj;
+      int q[3];
+      node_of(d, j, k, 0, q);
+      bool out = s.ind[g.idx(q[0], q[1], q[2])] > 0;
+      t.cls0[d][lid] = out ? 1 : 0;
+      int c = s.line_ptr[d][lid];
+      const int c1 = s.line_ptr[d][lid + 1];
+      for (int i = 0; i < n; i++) {
+        while (c < c1 && s.cross[d][c].left_loc < i) {
+          out = !out;
+          c++;
+        }
+        node_of(d, j, k, i, q);
+        if ((s.ind[g.idx(q[0], q[1], q[2])] > 0) != out)
+          return "node classification is not the line's first class flipped at its crossings";
+      }
+    }
     t.n_blocks[d] = (int)t.blk_hdr[d].size();
   }
 
@@ -249,6 +272,7 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
   for (int a = 0; a < 3; a++) {
     double h = g.h[a];
     p.hh[a] = h * h;
+    p.ihh[a] = 1.0 / (h * h);
     double bs[2] = {beta.p[0], beta.p[1]};
     for (int side = 0; side < 2; side++) {
       double b = bs[side];
@@ -257,6 +281,7 @@ void build_device_table(const SetupResult &s, const BetaModel &beta, double thet
       p.vdc[a][side][2] = b / (h * h);
     }
   }
+  return std::string();
 }
 
 }  // namespace mts

@@ -147,166 +147,11 @@ __device__ __forceinline__ void wave_solve(double (&a)[MMAX], double (&b)[MMAX],
   }
 }
 
-__device__ __forceinline__ double read_lane(double v, int k) {
-  const int lo = __builtin_amdgcn_readlane(__double2loint(v), k);
-  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), k);
-  return __hiloint2double(hi, lo);
-}
-
-// The sequential Thomas algorithm over the whole line, in the reference's
-// TDMA operation order (douglas_adi.cc:1784-1807), for lines the partitioned
-// solve would not handle stably (a non-diagonally-dominant row).  The rows
-// stay distributed as for wave_solve; the recurrence runs wave-uniformly:
-// row i's coefficients are read from the lane that owns it, and its factor
-// gamma_i and forward value are written back into that lane's a and d.  On
-// return d holds the solution (xs, xe as for wave_solve).
-template <int MMAX>
-__device__ __forceinline__ void wave_thomas(double (&a)[MMAX], const double (&b)[MMAX],
-                                            const double (&c)[MMAX], double (&d)[MMAX], int n,
-                                            int m, int lane, double &xs, double &xe) {
-  const int p = n / 2 > 64 ? 64 : n / 2;
-  double bet = 1.0, uprev = 0.0, cprev = 0.0;
-  bool first = true;
-  for (int k = 0; k < p; k++) {
-    int sk, mk;
-    segment(n, k, sk, mk);
-#pragma unroll
-    for (int r = 0; r < MMAX; r++) {
-      if (r < mk) {
-        const double ai = read_lane(a[r], k), bi = read_lane(b[r], k), di = read_lane(d[r], k);
-        double g = 0.0, u;
-        if (first) {
-          bet = bi;
-          u = di / bet;
-          first = false;
-        } else {
-          g = cprev / bet;
-          bet = bi - ai * g;
-          u = (di - ai * uprev) / bet;
-        }
-        if (lane == k) {
-          a[r] = g;
-          d[r] = u;
-        }
-        cprev = read_lane(c[r], k);
-        uprev = u;
-      }
-    }
-  }
-  double unext = 0.0, gnext = 0.0;
-  bool last = true;
-  for (int k = p - 1; k >= 0; k--) {
-    int sk, mk;
-    segment(n, k, sk, mk);
-#pragma unroll
-    for (int r = MMAX - 1; r >= 0; r--) {
-      if (r < mk) {
-        double u = read_lane(d[r], k);
-        if (!last) u -= gnext * unext;
-        last = false;
-        if (lane == k) d[r] = u;
-        gnext = read_lane(a[r], k);
-        unext = u;
-      }
-    }
-  }
-  xs = d[0];
-  xe = d[0];
-#pragma unroll
-  for (int r = 0; r < MMAX; r++)
-    if (r == m - 1) xe = d[r];
-}
-
 // value of local row r after wave_solve
 template <int MMAX>
 __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int m, double xs,
                                              double xe) {
   return r == 0 ? xs : (r == m - 1 ? xe : d[r]);
-}
-
-// ----------------------------------------------------------- line kernel
-template <int DIR, int MMAX, int FLAGS>
-__global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
-  constexpr int RHS = FLAGS & 3;
-  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
-  constexpr bool HAS_S = (RHS == kRhsDouglas);
-  constexpr bool TWO = HAS_S || !SOLVE;  // second LDS tile: S, or the output of explicit stages
-  constexpr bool SEQ = SOLVE && (FLAGS & kFlagSeq) != 0;
-  const DevGeom &g = p.g;
-  const int n = g.n[DIR];
-  const int ns = n + 1;
-  const int nj = LineMap<DIR>::nj(g);
-  const int j0 = blockIdx.x * kTile;
-  const int k = blockIdx.y + 1;
-
-  extern __shared__ double smem[];
-  double *tA = smem;
-  double *tB = smem + kTile * ns;
-  int8_t *tI = reinterpret_cast<int8_t *>(smem + (TWO ? 2 : 1) * kTile * ns);
-
-  for (int e = threadIdx.x; e < kTile * n; e += blockDim.x)
-    tile_load<DIR, HAS_S>(p, j0, k, e, tA, tB, tI);
-  __syncthreads();
-
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int nwave = blockDim.x >> 6;
-  int s, m;
-  segment(n, lane, s, m);
-  for (int l = wave; l < kTile; l += nwave) {
-    const int j = j0 + l;
-    if (j < 1 || j > nj - 2) continue;  // uniform per wavefront
-    const int8_t *li = tI + l * ns;
-    double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-    line_rows<DIR, MMAX, FLAGS>(p, j, k, tA + l * ns, tB + l * ns, li, s, m, a, b, c, d);
-    double xs = 0.0, xe = 0.0;
-    if (SOLVE) {
-      if (SEQ && p.mib.seq[k * nj + j])  // uniform per wavefront
-        wave_thomas<MMAX>(a, b, c, d, n, m, lane, xs, xe);
-      else
-        wave_solve<MMAX>(a, b, c, d, m, lane, xs, xe);
-    }
-    line_write<DIR, MMAX, FLAGS>(p, j, k, li, s, m, d, xs, xe,
-                                 (TWO && !HAS_S ? tB : tA) + l * ns);
-  }
-  __syncthreads();
-
-  const double *tO = TWO && !HAS_S ? tB : tA;
-  for (int e = threadIdx.x; e < kTile * n; e += blockDim.x) tile_store<DIR>(p, j0, k, e, tO);
-}
-
-template <int DIR, int MMAX, int FLAGS>
-hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
-  using M = LineMap<DIR>;
-  const DevGeom &g = a.g;
-  const int n = g.n[DIR];
-  const int nj = DIR == 0 ? g.n[1] : g.n[0];
-  const int nk = DIR == 2 ? g.n[1] : g.n[2];
-  (void)sizeof(M);
-  constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
-  size_t shm = (size_t)kTile * (n + 1) * sizeof(double) * (TWO ? 2 : 1) + (size_t)kTile * (n + 1);
-  dim3 grid((nj + kTile - 1) / kTile, nk - 2);
-  hipLaunchKernelGGL((line_kernel<DIR, MMAX, FLAGS>), grid, dim3(kBlock), shm, st, a);
-  return hipGetLastError();
-}
-
-template <int MMAX, int FLAGS>
-hipError_t launch_line_m(int dir, const LineArgs &a, hipStream_t st) {
-  if (dir == 0) return launch_line_t<0, MMAX, FLAGS>(a, st);
-  if (dir == 1) return launch_line_t<1, MMAX, FLAGS>(a, st);
-  return launch_line_t<2, MMAX, FLAGS>(a, st);
-}
-
-template <int FLAGS>
-hipError_t launch_line_f(int dir, const LineArgs &a, hipStream_t st) {
-  const int n = a.g.n[dir];
-  int p = n / 2;
-  if (p > 64) p = 64;
-  const int mmax = (n + p - 1) / p;
-  if (mmax <= 4) return launch_line_m<4, FLAGS>(dir, a, st);
-  if (mmax <= 8) return launch_line_m<8, FLAGS>(dir, a, st);
-  if (mmax <= 16) return launch_line_m<16, FLAGS>(dir, a, st);
-  return hipErrorInvalidValue;
 }
 
 // ------------------------------------------------ Douglas stage-1 RHS
@@ -450,27 +295,18 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
 }  // namespace
 
 // --------------------------------------------------------------- launchers
-// The sequential-solve variant is compiled separately so that its register
-// needs do not lower the occupancy of the common case (no such line).
-#define MTS_LINE_CASE(F)                                                       \
-  case (F):                                                                    \
-    return seq ? launch_line_f<(F) | kFlagSeq>(dir, a, s) : launch_line_f<(F)>(dir, a, s);
-
-hipError_t launch_line(int dir, int flags, const LineArgs &a, hipStream_t s) {
+hipError_t launch_line(int dir, int flags, const LineArgs &args, hipStream_t s) {
+  LineArgs a = args;
+  if (!make_line_part(a.g.n[dir], a.q, a.min_lanes)) return hipErrorInvalidValue;
   const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
-  switch (flags) {
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve)
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump)
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF)
-    MTS_LINE_CASE(kRhsDouglas | kFlagSolve)
-    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump)
-    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF)
-    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump)
-    case kRhsCN: return launch_line_f<kRhsCN>(dir, a, s);
-    default: return hipErrorInvalidValue;
-  }
+#define MTS_DL(D, LL) \
+  if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, a, s);
+  MTS_DL(0, 16) MTS_DL(0, 32) MTS_DL(0, 64)
+  MTS_DL(1, 16) MTS_DL(1, 32) MTS_DL(1, 64)
+  MTS_DL(2, 16) MTS_DL(2, 32) MTS_DL(2, 64)
+#undef MTS_DL
+  return hipErrorInvalidValue;
 }
-#undef MTS_LINE_CASE
 
 hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s) {
   dim3 grid((a.g.n[0] + 255) / 256, a.g.n[1], a.g.n[2]);

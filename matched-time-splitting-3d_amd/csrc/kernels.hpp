@@ -13,6 +13,14 @@ namespace mts {
 // One line stage along direction dir with the given flags.
 hipError_t launch_line(int dir, int flags, const LineArgs &a, hipStream_t s);
 
+// Per direction and lane-group size (line_kernels.hip); a.q already filled.
+#define MTS_DECL_DL(D, LL) \
+  hipError_t launch_line_dl_##D##_##LL(int flags, bool seq, const LineArgs &a, hipStream_t s);
+MTS_DECL_DL(0, 16) MTS_DECL_DL(0, 32) MTS_DECL_DL(0, 64)
+MTS_DECL_DL(1, 16) MTS_DECL_DL(1, 32) MTS_DECL_DL(1, 64)
+MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
+#undef MTS_DECL_DL
+
 // Douglas stage 1 right-hand side (see device_core.hpp).
 hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s);
 

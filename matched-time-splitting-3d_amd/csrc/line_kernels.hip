@@ -1,0 +1,349 @@
+// gfx950 line-stage kernels (see kernels.hip for the overview).  This file
+// is compiled once per line direction and lane-group size
+// (-DMTS_LINE_DIR=0|1|2 -DMTS_LINE_L=16|32|64), so the many template
+// instances build in parallel; each object defines one
+// launch_line_dl_<dir>_<L> entry used by launch_line.
+#include <hip/hip_runtime.h>
+
+#include "device_core.hpp"
+#include "kernels.hpp"
+
+#ifndef MTS_LINE_DIR
+#error "compile with -DMTS_LINE_DIR=0|1|2 -DMTS_LINE_L=16|32|64"
+#endif
+
+namespace mts {
+
+namespace {
+
+constexpr int kBlock = kBlockThreads;
+
+// ------------------------------------------------- group line solver
+// Lanes of a group of L lanes (16, 32 or 64; a DPP row when L = 16) hold the
+// segments of one line.  Shifts within the group return id where the source
+// lane lies outside it.
+template <int L>
+__device__ __forceinline__ double grp_up(double v, int s, int kseg, double id) {
+  if (L == 16) {
+    // row_shr:s -- lane i takes lane i - s of its 16-lane row
+    const int ctl = 0x110 | s;
+    int lo = 0, hi = 0;
+    switch (ctl) {
+#define MTS_DPP(c)                                                                        \
+  case c:                                                                                 \
+    lo = __builtin_amdgcn_update_dpp(__double2loint(id), __double2loint(v), c, 0xf, 0xf, false); \
+    hi = __builtin_amdgcn_update_dpp(__double2hiint(id), __double2hiint(v), c, 0xf, 0xf, false); \
+    break;
+      MTS_DPP(0x111) MTS_DPP(0x112) MTS_DPP(0x114) MTS_DPP(0x118)
+      default: break;
+    }
+    return __hiloint2double(hi, lo);
+  } else {
+    const double w = __shfl_up(v, s, L);
+    return kseg < s ? id : w;
+  }
+}
+template <int L>
+__device__ __forceinline__ double grp_down(double v, int s, int kseg, double id) {
+  if (L == 16) {
+    // row_shl:s -- lane i takes lane i + s of its 16-lane row
+    const int ctl = 0x100 | s;
+    int lo = 0, hi = 0;
+    switch (ctl) {
+      MTS_DPP(0x101) MTS_DPP(0x102) MTS_DPP(0x104) MTS_DPP(0x108)
+      default: break;
+    }
+    return __hiloint2double(hi, lo);
+  } else {
+    const double w = __shfl_down(v, s, L);
+    return kseg + s >= L ? id : w;
+  }
+}
+#undef MTS_DPP
+
+// 1 / x: hardware reciprocal refined by two Newton steps (within an ulp or
+// so of the correctly rounded quotient).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// Partitioned solve of the group's line: lane k holds rows [s_k, s_k + m)
+// of  a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i  (m >= 2, or m == 0 on idle
+// lanes; the first row of the first segment has a = 0 and the last row of
+// the last segment c = 0).  Each lane eliminates its segment down and up
+// keeping its first unknown as a parameter, the first unknown is folded
+// out with the next lane's first row, the one-equation-per-lane system in
+// the segment-end unknowns is solved by parallel cyclic reduction over the
+// group, and each lane back-substitutes.  On return x holds the solution.
+template <int L, int MMAX>
+__device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)[MMAX],
+                                            double (&c)[MMAX], double (&d)[MMAX], int m,
+                                            int kseg) {
+  double A1 = 0.0, B1 = 1.0, C1 = 0.0, D1 = 0.0;
+  double A2 = 0.0, C2 = 0.0, D2 = 0.0;
+  if (m >= 2) {
+    {
+      const double r = rcp_nr(b[1]);
+      a[1] = a[1] * r;
+      c[1] = c[1] * r;
+      d[1] = d[1] * r;
+      A2 = a[1];
+      C2 = c[1];
+      D2 = d[1];
+    }
+#pragma unroll
+    for (int i = 2; i < MMAX; i++) {
+      if (i < m) {
+        const double r = rcp_nr(b[i] - a[i] * c[i - 1]);
+        d[i] = (d[i] - a[i] * d[i - 1]) * r;
+        a[i] = -a[i] * a[i - 1] * r;
+        c[i] = c[i] * r;
+        A2 = a[i];
+        C2 = c[i];
+        D2 = d[i];
+      }
+    }
+#pragma unroll
+    for (int i = MMAX - 3; i >= 1; i--) {
+      if (i <= m - 3) {
+        d[i] = d[i] - c[i] * d[i + 1];
+        a[i] = a[i] - c[i] * a[i + 1];
+        c[i] = -c[i] * c[i + 1];
+      }
+    }
+    if (m == 2) {
+      A1 = a[0];
+      B1 = b[0];
+      C1 = c[0];
+      D1 = d[0];
+    } else {
+      A1 = a[0];
+      B1 = b[0] - c[0] * a[1];
+      C1 = -c[0] * c[1];
+      D1 = d[0] - c[0] * d[1];
+    }
+  }
+  const double A1n = grp_down<L>(A1, 1, kseg, 0.0);
+  const double B1n = grp_down<L>(B1, 1, kseg, 1.0);
+  const double C1n = grp_down<L>(C1, 1, kseg, 0.0);
+  const double D1n = grp_down<L>(D1, 1, kseg, 0.0);
+  double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0;
+  const double i1 = rcp_nr(B1);
+  if (m >= 2) {
+    const double in = rcp_nr(B1n);
+    ea = -A2 * A1 * i1;
+    eb = 1.0 - A2 * C1 * i1 - C2 * A1n * in;
+    ec = -C2 * C1n * in;
+    ed = D2 - A2 * D1 * i1 - C2 * D1n * in;
+  }
+#pragma unroll
+  for (int s = 1; s < L; s <<= 1) {
+    const double am = grp_up<L>(ea, s, kseg, 0.0), bm = grp_up<L>(eb, s, kseg, 1.0);
+    const double cm = grp_up<L>(ec, s, kseg, 0.0), dm = grp_up<L>(ed, s, kseg, 0.0);
+    const double ap = grp_down<L>(ea, s, kseg, 0.0), bp = grp_down<L>(eb, s, kseg, 1.0);
+    const double cp = grp_down<L>(ec, s, kseg, 0.0), dp = grp_down<L>(ed, s, kseg, 0.0);
+    const double k1 = ea * rcp_nr(bm), k2 = ec * rcp_nr(bp);
+    const double na = -am * k1;
+    const double nc = -cp * k2;
+    const double nb = eb - cm * k1 - ap * k2;
+    const double nd = ed - dm * k1 - dp * k2;
+    ea = na;
+    eb = nb;
+    ec = nc;
+    ed = nd;
+  }
+  const double xe = ed * rcp_nr(eb);
+  const double xprev = grp_up<L>(xe, 1, kseg, 0.0);
+  // (written as selects of computed values: conditional stores of one value
+  // to several elements get merged into one indexed store, which keeps the
+  // array in scratch memory)
+  const double xs = (D1 - A1 * xprev - C1 * xe) * i1;
+  const bool act = m >= 2;
+#pragma unroll
+  for (int i = 0; i < MMAX; i++) {
+    const double mid = d[i] - a[i] * xs - c[i] * xe;
+    double v = i == 0 ? xs : (i == m - 1 ? xe : mid);
+    d[i] = act && i < m ? v : d[i];
+  }
+}
+
+// The sequential Thomas algorithm in the reference's TDMA operation order
+// (douglas_adi.cc:1784-1807) over the whole line, rows 0 and n-1 included,
+// for the groups whose line is flagged (seq_line): the partitioned solve is
+// stable only for diagonally dominant lines.  The recurrence runs in every
+// lane of the group on rows gathered from their owner lanes; the owner keeps
+// gamma in a and the forward value, then the solution, in d.  Solved rows
+// become identity rows (a = c = 0, b = 1, d = x), which the partitioned
+// solve that follows returns unchanged.
+template <int L, int MMAX>
+__device__ __forceinline__ void group_thomas(double (&a)[MMAX], double (&b)[MMAX],
+                                             double (&c)[MMAX], double (&d)[MMAX],
+                                             const LinePart &q, int lane, int kseg,
+                                             bool seq_line, double x0, double xn) {
+  const int gb = lane - kseg;
+  double bet = 1.0, uprev = x0, cprev = 0.0;
+  for (int kk = 0; kk < q.p; kk++) {
+    const int mk = kk == q.p - 1 ? q.mlast : q.m;
+    const int src = gb + kk;
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      if (r < mk) {
+        const double ai = __shfl(a[r], src, 64), bi = __shfl(b[r], src, 64);
+        const double ci = __shfl(c[r], src, 64), di = __shfl(d[r], src, 64);
+        const double gam = cprev / bet;
+        bet = bi - ai * gam;
+        const double u = (di - ai * uprev) / bet;
+        if (seq_line && kseg == kk) {
+          a[r] = gam;
+          d[r] = u;
+        }
+        cprev = ci;
+        uprev = u;
+      }
+    }
+  }
+  double gnext = cprev / bet, unext = xn;  // row n-1: gamma, and its value
+  for (int kk = q.p - 1; kk >= 0; kk--) {
+    const int mk = kk == q.p - 1 ? q.mlast : q.m;
+    const int src = gb + kk;
+#pragma unroll
+    for (int r = MMAX - 1; r >= 0; r--) {
+      if (r < mk) {
+        const double ui = __shfl(d[r], src, 64), gi = __shfl(a[r], src, 64);
+        const double u = ui - gnext * unext;
+        if (seq_line && kseg == kk) d[r] = u;
+        gnext = gi;
+        unext = u;
+      }
+    }
+  }
+  if (seq_line) {
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      a[r] = 0.0;
+      b[r] = 1.0;
+      c[r] = 0.0;
+    }
+  }
+}
+
+// ----------------------------------------------------------- line kernel
+template <int DIR, int L, int MMAX, int FLAGS>
+__global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
+  constexpr int RHS = FLAGS & 3;
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  constexpr bool HAS_S = (RHS == kRhsDouglas);
+  constexpr bool TWO = HAS_S || !SOLVE;  // second LDS tile: S, or the output of explicit stages
+  constexpr bool SEQ = SOLVE && (FLAGS & kFlagSeq) != 0;
+  constexpr int LPW = 64 / L;          // lines per wavefront
+  constexpr int T = tile_lines(L);     // lines per tile (all solved at once)
+  const DevGeom &g = p.g;
+  const LinePart &q = p.q;
+  const int n = g.n[DIR];
+  const int nj = LineMap<DIR>::nj(g);
+  const int j0 = blockIdx.x * T;
+  const int k = blockIdx.y + 1;
+  const unsigned magic_n = magic_of(n);
+
+  extern __shared__ double smem[];
+  double *tA = smem;
+  double *tB = smem + T * q.lds_len;
+
+  for (int e = threadIdx.x; e < T * n; e += blockDim.x)
+    tile_load<DIR, HAS_S, T>(p, j0, k, e, magic_n, tA, tB);
+  __syncthreads();
+
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kseg = lane & (L - 1);
+  {
+    const int l = wave * LPW + lane / L;
+    const int j = j0 + l;
+    const bool valid = j >= 1 && j <= nj - 2;
+    const LaneSeg sg = lane_seg(q, kseg, valid);
+    double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+    line_rows<DIR, MMAX, FLAGS>(p, j, k, tA + l * q.lds_len, tB + l * q.lds_len, sg, a, b, c, d);
+    double x0 = 0.0, xn = 0.0;
+    if (valid) line_ends<DIR>(p, j, k, x0, xn);
+    if (SOLVE) {
+      if (SEQ) {
+        const bool seq_line = valid && p.mib.seq[k * nj + j] != 0;
+        if (__any(seq_line)) group_thomas<L, MMAX>(a, b, c, d, q, lane, kseg, seq_line, x0, xn);
+      }
+      // boundary rows are known: move their couplings to the right-hand side
+      const bool first = sg.m > 0 && sg.k == 0;
+      d[0] = first ? d[0] - a[0] * x0 : d[0];
+      a[0] = first ? 0.0 : a[0];
+      const bool lastseg = sg.m > 0 && sg.k == q.p - 1;
+#pragma unroll
+      for (int r = 0; r < MMAX; r++) {
+        const bool hit = lastseg && r == sg.m - 1;
+        d[r] = hit ? d[r] - c[r] * xn : d[r];
+        c[r] = hit ? 0.0 : c[r];
+      }
+      group_solve<L, MMAX>(a, b, c, d, sg.m, kseg);
+    }
+    if (valid)
+      line_write<DIR, MMAX, FLAGS>(p, j, k, sg, d, x0, xn, (TWO && !HAS_S ? tB : tA) + l * q.lds_len);
+  }
+  __syncthreads();
+
+  const double *tO = TWO && !HAS_S ? tB : tA;
+  for (int e = threadIdx.x; e < T * n; e += blockDim.x)
+    tile_store<DIR, T>(p, j0, k, e, magic_n, tO);
+}
+
+template <int DIR, int L, int MMAX, int FLAGS>
+hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
+  const DevGeom &g = a.g;
+  const int nj = DIR == 0 ? g.n[1] : g.n[0];
+  const int nk = DIR == 2 ? g.n[1] : g.n[2];
+  constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
+  constexpr int T = tile_lines(L);
+  const size_t shm = (size_t)T * a.q.lds_len * sizeof(double) * (TWO ? 2 : 1);
+  dim3 grid((nj + T - 1) / T, nk - 2);
+  hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS>), grid, dim3(kBlock), shm, st, a);
+  return hipGetLastError();
+}
+
+template <int FLAGS>
+hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
+  constexpr int D = MTS_LINE_DIR, LL = MTS_LINE_L;
+  if (mmax == 4) return launch_line_t<D, LL, 4, FLAGS>(a, st);
+  if (mmax == 8) return launch_line_t<D, LL, 8, FLAGS>(a, st);
+  if (mmax == 16) return launch_line_t<D, LL, 16, FLAGS>(a, st);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace
+
+#define MTS_CAT2(a, b, c) a##b##_##c
+#define MTS_CAT(a, b, c) MTS_CAT2(a, b, c)
+#define MTS_ENTRY MTS_CAT(launch_line_dl_, MTS_LINE_DIR, MTS_LINE_L)
+
+// The sequential-solve variant is compiled separately so that its register
+// needs do not lower the occupancy of the common case (no such line).
+#define MTS_LINE_CASE(F)                                                     \
+  case (F):                                                                  \
+    return seq ? launch_flags<(F) | kFlagSeq>(a.q.mmax, a, s)               \
+               : launch_flags<(F)>(a.q.mmax, a, s);
+
+hipError_t MTS_ENTRY(int flags, bool seq, const LineArgs &a, hipStream_t s) {
+  switch (flags) {
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF)
+    MTS_LINE_CASE(kRhsDouglas | kFlagSolve)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF)
+    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump)
+    case kRhsCN: return launch_flags<kRhsCN>(a.q.mmax, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace mts

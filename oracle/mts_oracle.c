@@ -226,67 +226,100 @@ static int gtsv(int n, const double *a, const double *b, const double *c, const 
   return 0;
 }
 
-/* Model of the GPU's line solve, operation for operation.  A line with a
- * row that is not diagonally dominant (|b| < |a| + |c|) is solved with the
- * Thomas algorithm.  Any other line: rows split into p = min(64, n/2)
- * balanced segments over 64 lanes, each segment eliminated down and up
- * keeping its first unknown as a parameter, folded to one equation per lane
- * in the segment-end unknowns, parallel cyclic reduction over the 64 lanes
- * (idle lanes are identity rows), then each segment back-substituted. */
-static void seg_of(int n, int lane, int *s, int *m) {
-  int p = n / 2;
-  if (p > 64) p = 64;
-  int q = n / p, rem = n - q * p;
-  if (lane < p) {
-    *m = q + (lane < rem ? 1 : 0);
-    *s = lane * q + (lane < rem ? lane : rem);
-  } else {
-    *m = 0;
-    *s = n;
+/* Model of the GPU's line solve, operation for operation (the GPU forms
+ * reciprocals with a refined hardware estimate where this model divides).
+ * A line with a row that is not diagonally dominant (|b| < |a| + |c|) is
+ * solved with the Thomas algorithm.  Any other line: the boundary rows 0 and
+ * n-1 (identity rows) are moved to the right-hand side; the interior rows
+ * 1..n-2 are split into p segments of m rows (the last one mlast rows), one
+ * per lane of a group of L lanes (split_line: the smallest of 16, 32, 64
+ * lanes and 4, 8, 16 rows per lane that fits, with no one-row segment); each
+ * segment is eliminated down and up keeping its first unknown as a
+ * parameter, folded to one equation per lane in the segment-end unknowns,
+ * solved by parallel cyclic reduction over the L lanes (idle lanes are
+ * identity rows), then back-substituted. */
+typedef struct {
+  int L, m, mlast, p;
+} Split;
+
+static int split_line(int n, Split *q) {
+  static const int kL[3] = {16, 32, 64}, kM[3] = {4, 8, 16};
+  const int ni = n - 2;
+  if (ni < 2 || n > 1024) return -1;
+  for (int li = 0; li < 3; li++) {
+    const int L = kL[li];
+    int m = (ni + L - 1) / L;
+    if (m < 2) m = 2;
+    int p = (ni + m - 1) / m;
+    int mlast = ni - (p - 1) * m;
+    if (mlast == 1) {
+      p -= 1;
+      mlast = m + 1;
+    }
+    if (p > L) continue;
+    const int mx = m > mlast ? m : mlast;
+    for (int mi = 0; mi < 3; mi++)
+      if (mx <= kM[mi]) {
+        q->L = L;
+        q->m = m;
+        q->mlast = mlast;
+        q->p = p;
+        return 0;
+      }
   }
+  return -1;
 }
 
 static int part_solve(int n, const double *A, const double *B, const double *C, const double *R,
                       double *U) {
-  enum { L = 64 };
+  enum { LMAX = 64 };
+  Split q;
   for (int i = 0; i < n; i++)
     if (fabs(B[i]) < fabs(A[i]) + fabs(C[i])) return mts_oracle_tdma(n, A, B, C, R, U);
+  if (split_line(n, &q) != 0) return -1;
+  /* line systems: rows 0 and n-1 carry the boundary values */
+  if (A[0] != 0.0 || B[0] != 1.0 || C[0] != 0.0 || A[n - 1] != 0.0 || B[n - 1] != 1.0 ||
+      C[n - 1] != 0.0)
+    return -1;
   double *w = (double *)malloc(sizeof(double) * (size_t)n * 3);
   if (!w) return -1;
   double *a = w, *c = w + n, *d = w + 2 * n;
-  double A1[L], B1[L], C1[L], D1[L], A2[L], B2[L], C2[L], D2[L];
-  double ea[L], eb[L], ec[L], ed[L], xs[L], xe[L];
-  int sv[L], mv[L];
+  double A1[LMAX], B1[LMAX], C1[LMAX], D1[LMAX], A2[LMAX], C2[LMAX], D2[LMAX];
+  double ea[LMAX], eb[LMAX], ec[LMAX], ed[LMAX], xe[LMAX];
+  const int L = q.L;
   for (int i = 0; i < n; i++) {
     a[i] = A[i];
     c[i] = C[i];
     d[i] = R[i];
   }
-  for (int ln = 0; ln < L; ln++) {
-    int s, m;
-    seg_of(n, ln, &s, &m);
-    sv[ln] = s;
-    mv[ln] = m;
-    A1[ln] = 0.0; B1[ln] = 1.0; C1[ln] = 0.0; D1[ln] = 0.0;
-    A2[ln] = 0.0; B2[ln] = 1.0; C2[ln] = 0.0; D2[ln] = 0.0;
-    if (m < 2) continue;
+  /* boundary rows are identity rows: their values are known */
+  const double x0 = R[0], xn = R[n - 1];
+  d[1] = d[1] - a[1] * x0;
+  a[1] = 0.0;
+  d[n - 2] = d[n - 2] - c[n - 2] * xn;
+  c[n - 2] = 0.0;
+  for (int k = 0; k < L; k++) {
+    A1[k] = 0.0; B1[k] = 1.0; C1[k] = 0.0; D1[k] = 0.0;
+    A2[k] = 0.0; C2[k] = 0.0; D2[k] = 0.0;
+    if (k >= q.p) continue;
+    const int s = 1 + k * q.m, m = k == q.p - 1 ? q.mlast : q.m;
     double *ra = a + s, *rc = c + s, *rd = d + s;
     const double *rb = B + s;
     double r = 1.0 / rb[1];
     ra[1] = ra[1] * r;
     rc[1] = rc[1] * r;
     rd[1] = rd[1] * r;
-    A2[ln] = ra[1];
-    C2[ln] = rc[1];
-    D2[ln] = rd[1];
+    A2[k] = ra[1];
+    C2[k] = rc[1];
+    D2[k] = rd[1];
     for (int i = 2; i < m; i++) {
       r = 1.0 / (rb[i] - ra[i] * rc[i - 1]);
       rd[i] = (rd[i] - ra[i] * rd[i - 1]) * r;
       ra[i] = -ra[i] * ra[i - 1] * r;
       rc[i] = rc[i] * r;
-      A2[ln] = ra[i];
-      C2[ln] = rc[i];
-      D2[ln] = rd[i];
+      A2[k] = ra[i];
+      C2[k] = rc[i];
+      D2[k] = rd[i];
     }
     for (int i = m - 3; i >= 1; i--) {
       rd[i] = rd[i] - rc[i] * rd[i + 1];
@@ -294,53 +327,54 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
       rc[i] = -rc[i] * rc[i + 1];
     }
     if (m == 2) {
-      A1[ln] = ra[0]; B1[ln] = rb[0]; C1[ln] = rc[0]; D1[ln] = rd[0];
+      A1[k] = ra[0]; B1[k] = rb[0]; C1[k] = rc[0]; D1[k] = rd[0];
     } else {
-      A1[ln] = ra[0];
-      B1[ln] = rb[0] - rc[0] * ra[1];
-      C1[ln] = -rc[0] * rc[1];
-      D1[ln] = rd[0] - rc[0] * rd[1];
+      A1[k] = ra[0];
+      B1[k] = rb[0] - rc[0] * ra[1];
+      C1[k] = -rc[0] * rc[1];
+      D1[k] = rd[0] - rc[0] * rd[1];
     }
   }
-  for (int ln = 0; ln < L; ln++) {
-    ea[ln] = 0.0; eb[ln] = 1.0; ec[ln] = 0.0; ed[ln] = 0.0;
-    if (mv[ln] < 2) continue;
+  for (int k = 0; k < L; k++) {
+    ea[k] = 0.0; eb[k] = 1.0; ec[k] = 0.0; ed[k] = 0.0;
+    if (k >= q.p) continue;
     double a1n = 0.0, b1n = 1.0, c1n = 0.0, d1n = 0.0;
-    if (ln + 1 < L) {
-      a1n = A1[ln + 1]; b1n = B1[ln + 1]; c1n = C1[ln + 1]; d1n = D1[ln + 1];
+    if (k + 1 < L) {
+      a1n = A1[k + 1]; b1n = B1[k + 1]; c1n = C1[k + 1]; d1n = D1[k + 1];
     }
-    double i1 = 1.0 / B1[ln], in = 1.0 / b1n;
-    ea[ln] = -A2[ln] * A1[ln] * i1;
-    eb[ln] = B2[ln] - A2[ln] * C1[ln] * i1 - C2[ln] * a1n * in;
-    ec[ln] = -C2[ln] * c1n * in;
-    ed[ln] = D2[ln] - A2[ln] * D1[ln] * i1 - C2[ln] * d1n * in;
+    const double i1 = 1.0 / B1[k], in = 1.0 / b1n;
+    ea[k] = -A2[k] * A1[k] * i1;
+    eb[k] = 1.0 - A2[k] * C1[k] * i1 - C2[k] * a1n * in;
+    ec[k] = -C2[k] * c1n * in;
+    ed[k] = D2[k] - A2[k] * D1[k] * i1 - C2[k] * d1n * in;
   }
   for (int st = 1; st < L; st <<= 1) {
-    double na[L], nb[L], nc[L], nd[L];
-    for (int ln = 0; ln < L; ln++) {
+    double na[LMAX], nb[LMAX], nc[LMAX], nd[LMAX];
+    for (int k = 0; k < L; k++) {
       double am = 0.0, bm = 1.0, cm = 0.0, dm = 0.0, ap = 0.0, bp = 1.0, cp = 0.0, dp = 0.0;
-      if (ln >= st) { am = ea[ln - st]; bm = eb[ln - st]; cm = ec[ln - st]; dm = ed[ln - st]; }
-      if (ln + st < L) { ap = ea[ln + st]; bp = eb[ln + st]; cp = ec[ln + st]; dp = ed[ln + st]; }
-      double k1 = ea[ln] / bm, k2 = ec[ln] / bp;
-      na[ln] = -am * k1;
-      nc[ln] = -cp * k2;
-      nb[ln] = eb[ln] - cm * k1 - ap * k2;
-      nd[ln] = ed[ln] - dm * k1 - dp * k2;
+      if (k >= st) { am = ea[k - st]; bm = eb[k - st]; cm = ec[k - st]; dm = ed[k - st]; }
+      if (k + st < L) { ap = ea[k + st]; bp = eb[k + st]; cp = ec[k + st]; dp = ed[k + st]; }
+      const double k1 = ea[k] * (1.0 / bm), k2 = ec[k] * (1.0 / bp);
+      na[k] = -am * k1;
+      nc[k] = -cp * k2;
+      nb[k] = eb[k] - cm * k1 - ap * k2;
+      nd[k] = ed[k] - dm * k1 - dp * k2;
     }
-    memcpy(ea, na, sizeof(na));
-    memcpy(eb, nb, sizeof(nb));
-    memcpy(ec, nc, sizeof(nc));
-    memcpy(ed, nd, sizeof(nd));
+    memcpy(ea, na, sizeof(double) * L);
+    memcpy(eb, nb, sizeof(double) * L);
+    memcpy(ec, nc, sizeof(double) * L);
+    memcpy(ed, nd, sizeof(double) * L);
   }
-  for (int ln = 0; ln < L; ln++) xe[ln] = ed[ln] / eb[ln];
-  for (int ln = 0; ln < L; ln++) {
-    int s = sv[ln], m = mv[ln];
-    if (m < 2) continue;
-    double xp = ln == 0 ? 0.0 : xe[ln - 1];
-    xs[ln] = (D1[ln] - A1[ln] * xp - C1[ln] * xe[ln]) / B1[ln];
-    U[s] = xs[ln];
-    for (int i = 1; i < m - 1; i++) U[s + i] = d[s + i] - a[s + i] * xs[ln] - c[s + i] * xe[ln];
-    U[s + m - 1] = xe[ln];
+  for (int k = 0; k < L; k++) xe[k] = ed[k] * (1.0 / eb[k]);
+  U[0] = x0;
+  U[n - 1] = xn;
+  for (int k = 0; k < q.p; k++) {
+    const int s = 1 + k * q.m, m = k == q.p - 1 ? q.mlast : q.m;
+    const double xp = k == 0 ? 0.0 : xe[k - 1];
+    const double xs = (D1[k] - A1[k] * xp - C1[k] * xe[k]) * (1.0 / B1[k]);
+    U[s] = xs;
+    for (int i = 1; i < m - 1; i++) U[s + i] = d[s + i] - a[s + i] * xs - c[s + i] * xe[k];
+    U[s + m - 1] = xe[k];
   }
   free(w);
   return 0;
@@ -354,10 +388,7 @@ int mts_oracle_line_solve(int n, const double *a, const double *b, const double 
                           const double *r, double *u) {
   if (n < 2) return -1;
   if (g_line_solver == MTS_ORACLE_SOLVE_PIVOT) return gtsv(n, a, b, c, r, u);
-  if (g_line_solver == MTS_ORACLE_SOLVE_PARTITIONED) {
-    if (n < 4 || n > 1024) return -1;
-    return part_solve(n, a, b, c, r, u);
-  }
+  if (g_line_solver == MTS_ORACLE_SOLVE_PARTITIONED) return part_solve(n, a, b, c, r, u);
   return mts_oracle_tdma(n, a, b, c, r, u);
 }
 

@@ -51,8 +51,10 @@ int mts_oracle_tdma(int n, const double *a, const double *b, const double *c, co
  *   THOMAS       the reference's TDMA (default),
  *   PIVOT        tridiagonal Gaussian elimination with partial pivoting
  *                (LAPACK dgtsv's algorithm): a conditioning probe,
- *   PARTITIONED  a CPU model of the GPU's partitioned line solve (64 segments,
- *                fold, cyclic reduction), operation for operation.
+ *   PARTITIONED  a CPU model of the GPU's line solve (segments per lane of a
+ *                16/32/64-lane group, fold, cyclic reduction; Thomas for a line
+ *                with a non-dominant row), operation for operation; it takes
+ *                line systems, whose rows 0 and n-1 are identity rows.
  * Not part of the reference; tests use them to tell rounding from error. */
 enum { MTS_ORACLE_SOLVE_THOMAS = 0, MTS_ORACLE_SOLVE_PIVOT = 1, MTS_ORACLE_SOLVE_PARTITIONED = 2 };
 void mts_oracle_set_line_solver(int kind);

@@ -1,7 +1,8 @@
 """Every global-memory and LDS index the per-step kernels form, checked on the
 host: mts_cpucheck runs the kernels' own per-node / per-lane functions
 (device_core.hpp, compiled for the host under AddressSanitizer) over buffers
-of exactly the device sizes for one step of each method."""
+of exactly the device sizes for one step of each method, over grids whose
+lines split across 16-, 32- and 64-lane groups."""
 import os
 import subprocess
 import time
@@ -18,6 +19,10 @@ GEOMS = {
     "cfg1like": ["65", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
     "torus": ["49", "torus", "0.5", "0.2", "const", "0.01", "1"],
     "bumpy": ["41", "bumpy", R0, "0.1", "5", "3", "const", "1000", "1"],
+    # anisotropic grids: line splits over 16-, 32- and 64-lane groups
+    "wide_x": ["600x20x18", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
+    "wide_y": ["18x300x22", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
+    "wide_z": ["20x18x1024", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
 }
 
 

@@ -106,6 +106,33 @@ def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
         u = ug
 
 
+WIDE = [(600, 20, 18), (18, 300, 22), (20, 18, 1024), (130, 66, 34)]
+
+
+@pytest.mark.parametrize("method", ["lod-ie", "lod-cn", "ts"])
+@pytest.mark.parametrize("shape", WIDE)
+def test_gpu_wide_lines_match_oracle(mts, oracle, ctx, shape, method):
+    """Anisotropic grids: lines split over 64-, 32- and 16-lane groups."""
+    print(f"case wide {shape} {method}", flush=True)
+    phys = mts.physics("sincos", 10, 1)
+    st = mts.Setup(mts.grid(shape), mts.surface("ellipsoid", 0.7, 0.5, 0.4, 1 - 2 ** -20),
+                   phys, mib=2)
+    op = oracle.OracleProblem.from_setup(st, 1, (10, 1), -1.0, 1.0)
+    dt = 1 / 64
+    inst = mts.Instance(ctx, st, method, dt)
+    inst.initialize(0.0)
+    u = inst.download()
+    t = 0.0
+    for s in range(2):
+        uo, _ = op.run(method, dt, t, 1, u)
+        t += dt
+        inst.step(t)
+        ug = inst.download()
+        assert rel_err(ug, uo) <= growth_tol(uo), f"step {s + 1}"
+        u = ug
+    inst.close()
+
+
 @pytest.mark.parametrize("n", [2, 3, 4, 5, 17, 64, 65, 128, 129, 257, 513, 1024])
 def test_gpu_batched_tdma(oracle, ctx, n):
     rng = np.random.default_rng(n)

@@ -30,8 +30,9 @@ def test_solvers_match_dense_on_dominant_systems(oracle, n):
     b = 2.2 + np.abs(a) + np.abs(c)
     b *= rng.choice([-1.0, 1.0], n)
     r = rng.uniform(-1, 1, n)
-    a[0] = 0.0
-    c[-1] = 0.0
+    # line systems: rows 0 and n-1 are identity rows holding boundary values
+    a[0] = c[0] = a[-1] = c[-1] = 0.0
+    b[0] = b[-1] = 1.0
     x = np.linalg.solve(np.diag(b) + np.diag(a[1:], -1) + np.diag(c[:-1], 1), r)
     for k in KINDS:
         assert rel_err(oracle.line_solve(k, a, b, c, r), x) < 5e-15, k
