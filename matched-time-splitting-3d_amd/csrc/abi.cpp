@@ -17,6 +17,7 @@
 #include "host_setup.hpp"
 #include "host_table.hpp"
 #include "kernels.hpp"
+#include "stage_records.hpp"
 
 namespace {
 
@@ -117,6 +118,13 @@ struct mts_instance {
   DevBuf<double> cr_val[3];
   DevBuf<uint8_t> seq[3];
   DevBuf<uint8_t> cls0[3];
+  // stage records per (direction, stage kind)
+  std::map<int, std::unique_ptr<DevBuf<double>>> recs;
+  static int rec_key(int d, int flags) { return d * 64 + (flags & ~(mts::kFlagEpiF | mts::kFlagSeq)); }
+  const double *rec_for(int d, int flags) const {
+    auto it = recs.find(rec_key(d, flags));
+    return it == recs.end() ? nullptr : it->second->p;
+  }
   DevBuf<int> spec;
   int nspec = 0;
   DevBuf<double> part_max, part_sq;
@@ -401,6 +409,37 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
     if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
     MTS_HIP(in->cls0[d].upload(t.cls0[d].data(), t.cls0[d].size(), st));
   }
+  {
+    using namespace mts;
+    std::vector<std::pair<int, int>> kinds;  // (direction, stage kind) this method launches
+    for (int d = 0; d < 3; d++) {
+      if (method == MTS_ADI) kinds.push_back({d, d == 0 ? kRhsCopy | kFlagSolve : kRhsDouglas | kFlagSolve});
+      else if (method == MTS_LOD_IE) kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
+      else if (method == MTS_LOD_CN) kinds.push_back({d, kRhsCN | kFlagSolve | kFlagLhsJump});
+      else if (d < 2) {
+        kinds.push_back({d, kRhsCN});
+        kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
+      } else {
+        kinds.push_back({d, kRhsTSZ | kFlagSolve | kFlagLhsJump});
+      }
+    }
+    for (auto &kd : kinds) {
+      std::vector<double> rec;
+      std::string rerr;
+      try {
+        rerr = build_stage_records(setup->res, t, kd.first, kd.second, in->theta_dt, dt, 0.5 * dt,
+                                   0.5 * dt, rec);
+      } catch (const std::bad_alloc &) {
+        return fail(MTS_E_NOMEM, "out of host memory building the stage records");
+      }
+      if (!rerr.empty()) return fail(MTS_E_ARG, rerr);
+      std::unique_ptr<DevBuf<double>> b(new DevBuf<double>());
+      MTS_HIP(b->upload(rec.data(), rec.size(), st));
+      // the host vector must outlive the asynchronous copy
+      MTS_HIP(hipStreamSynchronize(st));
+      in->recs[mts_instance::rec_key(kd.first, kd.second)] = std::move(b);
+    }
+  }
   in->nspec = (int)(t.spec.size() / mts::kSpecLen);
   MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size(), st));
   mts::DevGeom &dg = in->geom;
@@ -502,6 +541,15 @@ static mts::LineArgs base_args(mts_instance *in) {
   return a;
 }
 
+// One line stage with the stage records of its direction and kind.
+static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::LineArgs &args,
+                              hipStream_t st) {
+  mts::LineArgs a = args;
+  a.mib.rec = in->rec_for(dir, flags);
+  if (!a.mib.rec && in->mib[dir].hdr) return hipErrorInvalidValue;
+  return mts::launch_line(dir, flags, a, st);
+}
+
 int mts_step(mts_instance *in, double t_next) {
   if (!in) return fail(MTS_E_ARG, "null instance");
   MTS_HIP(hipSetDevice(in->ctx->device));
@@ -535,13 +583,13 @@ int mts_step(mts_instance *in, double t_next) {
     a.mib = in->mib[0];
     a.A = in->r.p;
     a.out = in->r.p;
-    MTS_HIP(timed(in, "adi_x", [&] { return launch_line(0, kRhsCopy | kFlagSolve, a, st); }));
+    MTS_HIP(timed(in, "adi_x", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve, a, st); }));
     a.mib = in->mib[1];
     a.S = in->u.p;
-    MTS_HIP(timed(in, "adi_y", [&] { return launch_line(1, kRhsDouglas | kFlagSolve, a, st); }));
+    MTS_HIP(timed(in, "adi_y", [&] { return line_launch(in, 1, kRhsDouglas | kFlagSolve, a, st); }));
     a.mib = in->mib[2];
     a.out = in->u.p;
-    MTS_HIP(timed(in, "adi_z", [&] { return launch_line(2, kRhsDouglas | kFlagSolve, a, st); }));
+    MTS_HIP(timed(in, "adi_z", [&] { return line_launch(in, 2, kRhsDouglas | kFlagSolve, a, st); }));
     MTS_HIP(timed(in, "faces", [&] {
       return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
     }));
@@ -553,16 +601,16 @@ int mts_step(mts_instance *in, double t_next) {
     a.A = in->u.p;
     a.out = in->u.p;
     a.mib = in->mib[0];
-    MTS_HIP(timed(in, "lod_x", [&] { return launch_line(0, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "lod_x", [&] { return line_launch(in, 0, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
     a.in_analytic = 1;
     a.e_in = e1;
     a.mib = in->mib[1];
-    MTS_HIP(timed(in, "lod_y", [&] { return launch_line(1, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "lod_y", [&] { return line_launch(in, 1, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[2];
     a.e_f1 = e1;
     a.c_f1 = dt;
     MTS_HIP(timed(in, "lod_z", [&] {
-      return launch_line(2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
+      return line_launch(in, 2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
     }));
     MTS_HIP(timed(in, "faces", [&] {
       return launch_faces(in->geom, in->phys, in->u.p, e1, dt, st);
@@ -577,11 +625,11 @@ int mts_step(mts_instance *in, double t_next) {
     // explicit (I + dt/2 L_x), (I + dt/2 L_y) with boundary values at t^{n+1/2}
     a.e_out = eh;
     a.mib = in->mib[0];
-    MTS_HIP(timed(in, "ts_ex", [&] { return launch_line(0, kRhsCN, a, st); }));
+    MTS_HIP(timed(in, "ts_ex", [&] { return line_launch(in, 0, kRhsCN, a, st); }));
     a.in_analytic = 1;
     a.e_in = eh;
     a.mib = in->mib[1];
-    MTS_HIP(timed(in, "ts_ey", [&] { return launch_line(1, kRhsCN, a, st); }));
+    MTS_HIP(timed(in, "ts_ey", [&] { return line_launch(in, 1, kRhsCN, a, st); }));
     // explicit z, sources at t^n and t^{n+1}, implicit z
     a.mib = in->mib[2];
     a.e_out = e1;
@@ -589,13 +637,13 @@ int mts_step(mts_instance *in, double t_next) {
     a.c_f1 = 0.5 * dt;
     a.e_f2 = e1;
     a.c_f2 = 0.5 * dt;
-    MTS_HIP(timed(in, "ts_z", [&] { return launch_line(2, kRhsTSZ | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "ts_z", [&] { return line_launch(in, 2, kRhsTSZ | kFlagSolve | kFlagLhsJump, a, st); }));
     // implicit y, x
     a.e_in = e1;
     a.mib = in->mib[1];
-    MTS_HIP(timed(in, "ts_iy", [&] { return launch_line(1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "ts_iy", [&] { return line_launch(in, 1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[0];
-    MTS_HIP(timed(in, "ts_ix", [&] { return launch_line(0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
+    MTS_HIP(timed(in, "ts_ix", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     MTS_HIP(timed(in, "faces", [&] {
       return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
     }));

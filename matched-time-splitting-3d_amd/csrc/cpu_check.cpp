@@ -21,6 +21,7 @@
 #include "device_core.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
+#include "stage_records.hpp"
 
 using namespace mts;
 
@@ -59,6 +60,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
       std::memset(tA.get(), 0, sizeof(double) * T * q.lds_len);
       std::memset(tB.get(), 0, sizeof(double) * T * q.lds_len);
       const int j0 = bx * T;
+      if (T * n > kTileElsPerThread * kBlockThreads) std::abort();
       for (int e = 0; e < T * n; e++)
         tile_load<DIR, HAS_S, T>(p, j0, k, e, magic_n, tA.get(), tB.get());
       for (int wave = 0; wave < 4; wave++)
@@ -67,19 +69,17 @@ void emulate_line(Emu &E, const LineArgs &p) {
             const int l = wave * LPW + lane / L;
             const int j = j0 + l;
             const bool valid = j >= 1 && j <= nj - 2;
+            const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
             const LaneSeg sg = lane_seg(q, kseg, valid);
             if (sg.m > 0 && (sg.m < 2 || sg.m > MMAX || sg.s < 1 || sg.s + sg.m > n - 1)) std::abort();
             double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-            line_rows<DIR, MMAX, FLAGS>(p, j, k, tA.get() + l * q.lds_len,
+            line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA.get() + l * q.lds_len,
                                         tB.get() + l * q.lds_len, sg, a, b, c, d);
             double x0 = 0.0, xn = 0.0;
             if (valid) line_ends<DIR>(p, j, k, x0, xn);
-            if (SOLVE && valid && p.mib.seq) {  // the kernel's per-line solver choice
-              volatile uint8_t sq = p.mib.seq[k * nj + j];
-              (void)sq;
-            }
+            (void)SOLVE;
             if (valid)
-              line_write<DIR, MMAX, FLAGS>(p, j, k, sg, d, x0, xn,
+              line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
                                            (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len);
           }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
@@ -91,6 +91,11 @@ template <int FLAGS>
 void line(Emu &E, int dir, const LineArgs &args) {
   LineArgs p = args;
   if (!make_line_part(p.g.n[dir], p.q, p.min_lanes)) std::abort();
+  std::vector<double> rec;
+  const std::string rerr =
+      build_stage_records(*E.s, E.t, dir, FLAGS, p.theta_dt, p.dt, 0.5 * p.dt, 0.5 * p.dt, rec);
+  if (!rerr.empty()) std::abort();
+  p.mib.rec = rec.data();
   auto run = [&](auto lc, auto mc) {
     constexpr int LL = decltype(lc)::value;
     constexpr int MM = decltype(mc)::value;

@@ -384,50 +384,88 @@ MTS_HD unsigned magic_of(int n) {
   return (unsigned)((0x100000000ull + (unsigned long long)n - 1) / (unsigned long long)n);
 }
 
+// Tile element e: its global index, LDS slot, line and row (ok: the element
+// exists and its line is inside the grid).
+struct TileAt {
+  size_t gi;
+  int sl, j, r;
+  bool ok;
+};
+template <int DIR, int T>
+MTS_HD TileAt tile_at(const LineArgs &p, int j0, int k, int e, unsigned magic_n) {
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const int n = g.n[DIR];
+  TileAt t;
+  int l;
+  tile_elem<DIR, T>(e, n, magic_n, l, t.r);
+  t.j = j0 + l;
+  t.ok = e < T * n && t.j < M::nj(g);
+  t.gi = M::base(g, t.j, k) + (size_t)t.r * M::stride(g);
+  t.sl = l * p.q.lds_len + lds_slot(p.q, t.r);
+  return t;
+}
+// The value staged for an input element: the analytic boundary value on the
+// line's end rows when the stage asks for it, else what was read.
+template <int DIR>
+MTS_HD double tile_in_value(const LineArgs &p, const TileAt &t, int k, double v) {
+  const int n = p.g.n[DIR];
+  if (p.in_analytic && (t.r == 0 || t.r == n - 1)) {
+    int ix, iy, iz;
+    LineMap<DIR>::node(t.j, k, t.r, ix, iy, iz);
+    v = outer_u(p.g, p.e_in, ix, iy, iz);
+  }
+  return v;
+}
+// Elements per thread that cover a tile: T * n <= 17 * 256 for every split
+// make_line_part returns.
+constexpr int kTileElsPerThread = 17;
+
 // Stage one element of the tile's input field(s) in LDS.
 template <int DIR, bool HAS_S, int T>
 MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, unsigned magic_n, double *tA,
                       double *tB) {
-  using M = LineMap<DIR>;
-  const DevGeom &g = p.g;
-  const int n = g.n[DIR];
-  int l, r;
-  tile_elem<DIR, T>(e, n, magic_n, l, r);
-  const int j = j0 + l;
-  if (j >= M::nj(g)) return;
-  const size_t gi = M::base(g, j, k) + (size_t)r * M::stride(g);
-  double va = p.A[gi];
-  if (p.in_analytic && (r == 0 || r == n - 1)) {
-    int ix, iy, iz;
-    M::node(j, k, r, ix, iy, iz);
-    va = outer_u(g, p.e_in, ix, iy, iz);
-  }
-  const int sl = l * p.q.lds_len + lds_slot(p.q, r);
-  tA[sl] = va;
-  if (HAS_S) tB[sl] = p.S[gi];
+  const TileAt t = tile_at<DIR, T>(p, j0, k, e, magic_n);
+  if (!t.ok) return;
+  tA[t.sl] = tile_in_value<DIR>(p, t, k, p.A[t.gi]);
+  if (HAS_S) tB[t.sl] = p.S[t.gi];
 }
 
 // Write one element of the tile's result back (interior lines only).
 template <int DIR, int T>
 MTS_HD void tile_store(const LineArgs &p, int j0, int k, int e, unsigned magic_n,
                        const double *tO) {
-  using M = LineMap<DIR>;
-  const DevGeom &g = p.g;
-  const int n = g.n[DIR];
-  int l, r;
-  tile_elem<DIR, T>(e, n, magic_n, l, r);
-  const int j = j0 + l;
-  if (j < 1 || j > M::nj(g) - 2) return;
-  p.out[M::base(g, j, k) + (size_t)r * M::stride(g)] = tO[l * p.q.lds_len + lds_slot(p.q, r)];
+  const TileAt t = tile_at<DIR, T>(p, j0, k, e, magic_n);
+  if (!t.ok || t.j < 1 || t.j > LineMap<DIR>::nj(p.g) - 2) return;
+  p.out[t.gi] = tO[t.sl];
+}
+
+// Per-line table entries a lane needs, read once up front.
+struct LineMeta {
+  int bb, be;    // block range
+  bool out0;     // first node outside
+  bool seq;      // solved with the sequential Thomas algorithm
+};
+MTS_HD LineMeta line_meta(const DevMib &mib, int lid, bool valid) {
+  LineMeta m;
+  m.bb = m.be = 0;
+  m.out0 = false;
+  m.seq = false;
+  if (valid) {
+    m.bb = mib.blk_ptr[lid];
+    m.be = mib.blk_ptr[lid + 1];
+    m.out0 = mib.cls0[lid] != 0;
+    m.seq = mib.seq != nullptr && mib.seq[lid] != 0;
+  }
+  return m;
 }
 
 // Classification (true = outside) of row r of line lid: the line's first
 // node class flipped at every crossing left of r (an irregular block holds
 // one crossing at ll, a corner pair two, at ll and ll + 1).
-MTS_HD bool row_outside(const DevMib &mib, int lid, int r) {
-  bool o = mib.cls0[lid] != 0;
-  const int bb = mib.blk_ptr[lid], be = mib.blk_ptr[lid + 1];
-  for (int bi = bb; bi < be; bi++) {
+MTS_HD bool row_outside(const DevMib &mib, const LineMeta &lm, int r) {
+  bool o = lm.out0;
+  for (int bi = lm.bb; bi < lm.be; bi++) {
     const BlockHdr h = mib.hdr[bi];
     if (h.ll < r) o = !o;
     if (h.type == kBlkCor && h.ll + 1 < r) o = !o;
@@ -435,10 +473,9 @@ MTS_HD bool row_outside(const DevMib &mib, int lid, int r) {
   return o;
 }
 // Classification bits of rows s .. s + 31 (bit t set: row s + t outside).
-MTS_HD unsigned rows_outside(const DevMib &mib, int lid, int s) {
-  unsigned mask = mib.cls0[lid] ? 0xffffffffu : 0u;
-  const int bb = mib.blk_ptr[lid], be = mib.blk_ptr[lid + 1];
-  for (int bi = bb; bi < be; bi++) {
+MTS_HD unsigned rows_outside(const DevMib &mib, const LineMeta &lm, int s) {
+  unsigned mask = lm.out0 ? 0xffffffffu : 0u;
+  for (int bi = lm.bb; bi < lm.be; bi++) {
     const BlockHdr h = mib.hdr[bi];
     const int nf = h.type == kBlkCor ? 2 : 1;
     for (int f = 0; f < nf; f++) {
@@ -459,8 +496,8 @@ struct BlockRhs {
   double v0, v1, v2;
 };
 template <int DIR, int FLAGS>
-MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, int lid, const BlockHdr h,
-                          const double *la, const double *lb) {
+MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, const LineMeta &lm,
+                          const BlockHdr h, const double *la, const double *lb) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool LJ = (FLAGS & kFlagLhsJump) != 0;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
@@ -508,7 +545,7 @@ MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, int lid, const BlockH
       for (int r = 0; r < 2; r++) {
         int ix, iy, iz;
         M::node(j, k, ll + r, ix, iy, iz);
-        const bool out = row_outside(p.mib, lid, ll + r);
+        const bool out = row_outside(p.mib, lm, ll + r);
         bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
         bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
       }
@@ -555,7 +592,7 @@ MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, int lid, const BlockH
       for (int r = 0; r < 3; r++) {
         int ix, iy, iz;
         M::node(j, k, ll + r, ix, iy, iz);
-        const bool out = row_outside(p.mib, lid, ll + r);
+        const bool out = row_outside(p.mib, lm, ll + r);
         bv[r] = bv[r] + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
         bv[r] = bv[r] + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
       }
@@ -583,22 +620,54 @@ MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, int lid, const BlockH
   return r;
 }
 
+// The block's five input rows ll-1 .. ll+3 from LDS (0 past the line's end).
+template <int FLAGS>
+MTS_HD void block_inputs(const LineArgs &p, int ll, const double *la, const double *lb,
+                         double (&fa)[5], double (&fs)[5]) {
+  constexpr bool HAS_S = (FLAGS & 3) == kRhsDouglas;
+#pragma unroll
+  for (int t = 0; t < 5; t++) {
+    const int r = ll - 1 + t;
+    const bool in = r <= p.q.n - 1;
+    const int sl = lds_slot(p.q, r);
+    fa[t] = in ? la[sl] : 0.0;
+    fs[t] = HAS_S && in ? lb[sl] : 0.0;
+  }
+}
+// Right-hand side of one patched row from its stage record.
+template <int FLAGS>
+MTS_HD double block_row_rhs(const LineArgs &p, const double *w, const double (&fa)[5],
+                            const double (&fs)[5]) {
+  constexpr int RHS = FLAGS & 3;
+  double v = w[kRecWA] * fa[0];
+#pragma unroll
+  for (int t = 1; t < 5; t++) v += w[kRecWA + t] * fa[t];
+  if (RHS == kRhsDouglas) {
+#pragma unroll
+    for (int t = 0; t < 5; t++) v += w[kRecWS + t] * fs[t];
+  }
+  v += p.e_jump * w[kRecK];
+  if (RHS == kRhsTSZ) {
+    v += p.e_f1 * w[kRecK1];
+    v += p.e_f2 * w[kRecK2];
+  }
+  return v;
+}
+
 // Rows [s, s + m) of line (j, k): tridiagonal coefficients and right-hand
 // side of the stage, including the line's MIB blocks (the reference's
 // per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
 // couplings of rows 1 and n-2 to the boundary rows stay in a and c.
 // la/lb: the line's input field(s) staged in LDS (slot layout of LinePart).
 template <int DIR, int MMAX, int FLAGS>
-MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const double *lb,
-                      const LaneSeg &sg, double (&a)[MMAX], double (&b)[MMAX],
-                      double (&c)[MMAX], double (&d)[MMAX]) {
+MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const double *la,
+                      const double *lb, const LaneSeg &sg, double (&a)[MMAX],
+                      double (&b)[MMAX], double (&c)[MMAX], double (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
-  constexpr bool LJ = (FLAGS & kFlagLhsJump) != 0;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   using M = LineMap<DIR>;
   const DevGeom &g = p.g;
   const LinePart &q = p.q;
-  const int nj = M::nj(g);
   const double tdt = p.theta_dt;
   const double mtdt = -tdt;
   const int s = sg.s, m = sg.m;
@@ -610,8 +679,7 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const d
     d[r] = 0.0;
   }
   if (m == 0) return;
-  const int lid = k * nj + j;
-  const unsigned cm = rows_outside(p.mib, lid, s);
+  const unsigned cm = rows_outside(p.mib, lm, s);
   const int base = 1 + sg.k * q.stride;
   // the rows just before and after the segment
   const int sl_prev = sg.k == 0 ? 0 : base - q.stride + q.m - 1;
@@ -653,15 +721,16 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const d
   }
 
   // ---- MIB blocks of this line
-  const int bb = p.mib.blk_ptr[lid], be = p.mib.blk_ptr[lid + 1];
-  for (int bi = bb; bi < be; bi++) {
+  for (int bi = lm.bb; bi < lm.be; bi++) {
     const BlockHdr h = p.mib.hdr[bi];
     const int ll = h.ll;
     const int nr = (h.type == kBlkIrr) ? 2 : 3;
     if (ll + nr - 1 < s || ll >= s + m) continue;
-    const BlockRhs br = block_rhs<DIR, FLAGS>(p, j, k, lid, h, la, lb);
-    const double *lhs = p.mib.val + h.off + (h.type == kBlkIrr ? kIrrLhs : kCorLhs);
-    double bv[3] = {br.v0, br.v1, br.v2};
+    const double *rec = p.mib.rec + (size_t)bi * kRecLen;
+    double fa[5], fs[5];
+    block_inputs<FLAGS>(p, ll, la, lb, fa, fs);
+    double bv[3];
+    for (int r = 0; r < 3; r++) bv[r] = r < nr ? block_row_rhs<FLAGS>(p, rec + r * kRecRow, fa, fs) : 0.0;
     // The right-hand side goes in as a select of register values; a select
     // between a global load and an array element would be turned into a load
     // through a selected pointer, which keeps the array in scratch memory.
@@ -671,9 +740,10 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const d
       const int rr = s + r - ll;
       const bool hit = r < m && rr >= 0 && rr < nr;
       if (SOLVE && hit) {
-        a[r] = lhs[rr * 3 + 0];
-        b[r] = lhs[rr * 3 + 1];
-        c[r] = lhs[rr * 3 + 2];
+        const double *lhs = rec + rr * kRecRow + kRecLhs;
+        a[r] = lhs[0];
+        b[r] = lhs[1];
+        c[r] = lhs[2];
       }
       const double dv = rr == 0 ? bv0 : (rr == 1 ? bv1 : bv2);
       d[r] = hit ? dv : d[r];
@@ -684,15 +754,13 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const double *la, const d
 // Epilogue of the lane's rows into the LDS output tile; the group's lane 0
 // also writes the two boundary rows (x0, xn: their values).
 template <int DIR, int MMAX, int FLAGS>
-MTS_HD void line_write(const LineArgs &p, int j, int k, const LaneSeg &sg,
+MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, const LaneSeg &sg,
                        const double (&x)[MMAX], double x0, double xn, double *lo) {
   constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
   using M = LineMap<DIR>;
   const LinePart &q = p.q;
-  const int nj = M::nj(p.g);
-  const int lid = k * nj + j;
   if (sg.m > 0) {
-    const unsigned cm = EPI ? rows_outside(p.mib, lid, sg.s) : 0u;
+    const unsigned cm = EPI ? rows_outside(p.mib, lm, sg.s) : 0u;
     const int base = 1 + sg.k * q.stride;
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
@@ -712,9 +780,9 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LaneSeg &sg,
     if (EPI) {
       int ix, iy, iz;
       M::node(j, k, 0, ix, iy, iz);
-      x0 = x0 + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lid, 0), ix, iy, iz);
+      x0 = x0 + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lm, 0), ix, iy, iz);
       M::node(j, k, n - 1, ix, iy, iz);
-      xn = xn + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lid, n - 1), ix, iy, iz);
+      xn = xn + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lm, n - 1), ix, iy, iz);
     }
     lo[0] = x0;
     lo[1 + q.p * q.stride] = xn;

@@ -36,6 +36,20 @@ enum : int {
   kSpecLen = 10,
 };
 
+// Stage record of a block (kRecLen doubles, one kRecRow group per patched
+// row): the patched row's right-hand side of one stage kind written as a
+// linear form in the block's five input rows ll-1 .. ll+3 (A: the stage
+// input, S: the second input of a Douglas stage) and in the time factors,
+//   d_r = sum_t WA[t] A(ll-1+t) + sum_t WS[t] S(ll-1+t)
+//         + e_jump K + e_f1 K1 + e_f2 K2,
+// then the row's tridiagonal coefficients (a, b, c) after Convert2Tri.  The
+// coefficients are derived on the host from the reference-order block
+// computation (block_rhs) evaluated on unit inputs.
+enum : int {
+  kRecWA = 0, kRecWS = 5, kRecK = 10, kRecK1 = 11, kRecK2 = 12, kRecLhs = 13,
+  kRecRow = 16, kRecLen = 48,
+};
+
 // Block header (4 ints): type, ll, first crossing index, offset into blk_val.
 struct BlockHdr {
   int type, ll, c0, off;
@@ -84,6 +98,7 @@ struct DevMib {
   const double *cr;
   const uint8_t *seq;
   const uint8_t *cls0;
+  const double *rec;     // stage records of the launching stage kind (kRecLen per block)
 };
 
 // How a line of n nodes is split over the lanes of one lane group.  Rows 0

@@ -298,6 +298,7 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
 hipError_t launch_line(int dir, int flags, const LineArgs &args, hipStream_t s) {
   LineArgs a = args;
   if (!make_line_part(a.g.n[dir], a.q, a.min_lanes)) return hipErrorInvalidValue;
+  if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
   const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
 #define MTS_DL(D, LL) \
   if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, a, s);

@@ -253,8 +253,31 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   double *tA = smem;
   double *tB = smem + T * q.lds_len;
 
-  for (int e = threadIdx.x; e < T * n; e += blockDim.x)
-    tile_load<DIR, HAS_S, T>(p, j0, k, e, magic_n, tA, tB);
+  // stage the tile: kChunk loads in flight per thread, then their LDS stores
+  // (chunks one after the other, so the loads' registers stay bounded)
+  constexpr int kChunk = 6;
+#pragma unroll 1
+  for (int u0 = 0; u0 < kTileElsPerThread; u0 += kChunk) {
+    double va[kChunk], vb[kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; c++) {
+      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
+      va[c] = 0.0;
+      vb[c] = 0.0;
+      if (t.ok) {
+        va[c] = p.A[t.gi];
+        if (HAS_S) vb[c] = p.S[t.gi];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kChunk; c++) {
+      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
+      if (t.ok) {
+        tA[t.sl] = tile_in_value<DIR>(p, t, k, va[c]);
+        if (HAS_S) tB[t.sl] = vb[c];
+      }
+    }
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
@@ -264,15 +287,16 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     const int l = wave * LPW + lane / L;
     const int j = j0 + l;
     const bool valid = j >= 1 && j <= nj - 2;
+    const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
     const LaneSeg sg = lane_seg(q, kseg, valid);
     double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-    line_rows<DIR, MMAX, FLAGS>(p, j, k, tA + l * q.lds_len, tB + l * q.lds_len, sg, a, b, c, d);
+    line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len, sg, a, b, c,
+                                d);
     double x0 = 0.0, xn = 0.0;
     if (valid) line_ends<DIR>(p, j, k, x0, xn);
     if (SOLVE) {
       if (SEQ) {
-        const bool seq_line = valid && p.mib.seq[k * nj + j] != 0;
-        if (__any(seq_line)) group_thomas<L, MMAX>(a, b, c, d, q, lane, kseg, seq_line, x0, xn);
+        if (__any(lm.seq)) group_thomas<L, MMAX>(a, b, c, d, q, lane, kseg, lm.seq, x0, xn);
       }
       // boundary rows are known: move their couplings to the right-hand side
       const bool first = sg.m > 0 && sg.k == 0;
@@ -288,13 +312,26 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
       group_solve<L, MMAX>(a, b, c, d, sg.m, kseg);
     }
     if (valid)
-      line_write<DIR, MMAX, FLAGS>(p, j, k, sg, d, x0, xn, (TWO && !HAS_S ? tB : tA) + l * q.lds_len);
+      line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
+                                   (TWO && !HAS_S ? tB : tA) + l * q.lds_len);
   }
   __syncthreads();
 
   const double *tO = TWO && !HAS_S ? tB : tA;
-  for (int e = threadIdx.x; e < T * n; e += blockDim.x)
-    tile_store<DIR, T>(p, j0, k, e, magic_n, tO);
+#pragma unroll 1
+  for (int u0 = 0; u0 < kTileElsPerThread; u0 += kChunk) {
+    double vo[kChunk];
+#pragma unroll
+    for (int c = 0; c < kChunk; c++) {
+      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
+      vo[c] = t.ok ? tO[t.sl] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kChunk; c++) {
+      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
+      if (t.ok && t.j >= 1 && t.j <= nj - 2) p.out[t.gi] = vo[c];
+    }
+  }
 }
 
 template <int DIR, int L, int MMAX, int FLAGS>
