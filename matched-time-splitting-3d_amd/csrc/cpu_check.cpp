@@ -61,8 +61,28 @@ void emulate_line(Emu &E, const LineArgs &p) {
       std::memset(tB.get(), 0, sizeof(double) * T * q.lds_len);
       const int j0 = bx * T;
       if (T * n > kTileElsPerThread * kBlockThreads) std::abort();
-      for (int e = 0; e < T * n; e++)
-        tile_load<DIR, HAS_S, T>(p, j0, k, e, magic_n, tA.get(), tB.get());
+      const int nh = DIR == 0 ? (n + 1) >> 1 : 1;
+      const unsigned magic_nh = magic_of(nh);
+      (void)magic_n;
+      // the kernel's paired 16-byte loads: both doubles are read whenever the
+      // first element exists
+      for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
+        const TilePair t = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
+        if (!t.e0.ok) continue;
+        if (t.e0.gi % 2 != 0) std::abort();  // 16-byte alignment
+        const double a0 = p.A[t.e0.gi], a1 = p.A[t.e0.gi + 1];
+        double s0 = 0.0, s1 = 0.0;
+        if (HAS_S) {
+          s0 = p.S[t.e0.gi];
+          s1 = p.S[t.e0.gi + 1];
+        }
+        tA[t.e0.sl] = tile_in_value<DIR>(p, t.e0, k, a0);
+        if (HAS_S) tB[t.e0.sl] = s0;
+        if (t.e1.ok) {
+          tA[t.e1.sl] = tile_in_value<DIR>(p, t.e1, k, a1);
+          if (HAS_S) tB[t.e1.sl] = s1;
+        }
+      }
       for (int wave = 0; wave < 4; wave++)
           for (int lane = 0; lane < 64; lane++) {
             const int kseg = lane & (L - 1);
@@ -83,7 +103,14 @@ void emulate_line(Emu &E, const LineArgs &p) {
                                            (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len);
           }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
-      for (int e = 0; e < T * n; e++) tile_store<DIR, T>(p, j0, k, e, magic_n, tO);
+      for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
+        const TilePair t = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
+        const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
+        const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
+        const double v0 = t.e0.ok ? tO[t.e0.sl] : 0.0, v1 = t.e1.ok ? tO[t.e1.sl] : 0.0;
+        if (w0) p.out[t.e0.gi] = v0;
+        if (w1) p.out[t.e1.gi] = v1;
+      }
     }
 }
 

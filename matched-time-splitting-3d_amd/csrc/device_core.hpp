@@ -366,6 +366,38 @@ struct LineMap {
   }
 };
 
+// The source f at the rows of one line.  Along a z line x and y are fixed,
+// so f is sin z (inside) or cos z (outside) times a factor of the line:
+// f = e sz [(3 beta - 1) sx sy - beta_x cx sy - beta_y sx cy] inside and
+// f = e (3 beta+ - 1) cx cy cz outside (the factors of source(), regrouped).
+// Other directions evaluate source() per row.
+template <int DIR>
+struct LineSource {
+  double fin = 0.0, fout = 0.0;
+  MTS_HD LineSource(const PhysConst &P, const DevGeom &g, int j, int k) {
+    if (DIR == 2) {
+      const double sx = tabv(g, 0, kTabSin, j), sy = tabv(g, 1, kTabSin, k);
+      const double cx = tabv(g, 0, kTabCos, j), cy = tabv(g, 1, kTabCos, k);
+      fout = P.f_out * cx * cy;
+      if (P.beta_kind == 0) {
+        fin = P.f_in * sx * sy;
+      } else {
+        const double b = P.b_in * (1.0 + 0.5 * sx * cy);
+        const double bx = 0.5 * P.b_in * cx * cy;
+        const double by = -0.5 * P.b_in * sx * sy;
+        fin = (3.0 * b - 1.0) * sx * sy - bx * cx * sy - by * sx * cy;
+      }
+    }
+  }
+  MTS_HD double at(const PhysConst &P, const DevGeom &g, double e, bool out, int j, int k,
+                   int i) const {
+    if (DIR == 2) return e * (out ? fout * tabv(g, 2, kTabCos, i) : fin * tabv(g, 2, kTabSin, i));
+    int ix, iy, iz;
+    LineMap<DIR>::node(j, k, i, ix, iy, iz);
+    return source(P, g, e, out, ix, iy, iz);
+  }
+};
+
 // Tile element e -> (line l, row r) for a tile of T lines.  y/z lines: T
 // consecutive lines (one segment of an x row) per line row; x lines: one
 // contiguous line after the other (l = e / n by a multiply-high with
@@ -420,6 +452,53 @@ MTS_HD double tile_in_value(const LineArgs &p, const TileAt &t, int k, double v)
 // Elements per thread that cover a tile: T * n <= 17 * 256 for every split
 // make_line_part returns.
 constexpr int kTileElsPerThread = 17;
+
+// Tile elements in pairs that share one 16-byte access: two neighbouring
+// lines of one row (y/z lines; the tile's first line is even) or two
+// neighbouring rows of one line (x lines, each line taken as ceil(n/2)
+// pairs).  The second element of a pair may not exist (a line past the grid,
+// the row past an odd line's end); the 16 bytes are then still inside the
+// padded x row (the pitch is a multiple of 16 doubles and above n when n is
+// odd), so the access stays inside the field.
+struct TilePair {
+  TileAt e0, e1;
+};
+// Pairs per thread that cover a tile: T * ceil(n / 2) <= 9 * 256.
+constexpr int kTilePairsPerThread = 9;
+template <int DIR, int T>
+MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
+  using M = LineMap<DIR>;
+  const DevGeom &g = p.g;
+  const int n = g.n[DIR];
+  TilePair t;
+  int l0, l1, r0, r1;
+  bool in;
+  if (DIR == 0) {
+    const int nh = (n + 1) >> 1;
+    l0 = (int)(((unsigned long long)(unsigned)pe * magic_nh) >> 32);
+    r0 = 2 * (pe - l0 * nh);
+    l1 = l0;
+    r1 = r0 + 1;
+    in = pe < T * nh;
+  } else {
+    l0 = 2 * (pe & (T / 2 - 1));
+    r0 = pe / (T / 2);
+    l1 = l0 + 1;
+    r1 = r0;
+    in = pe < (T / 2) * n;
+  }
+  t.e0.r = r0;
+  t.e0.j = j0 + l0;
+  t.e0.ok = in && t.e0.j < M::nj(g);
+  t.e0.gi = M::base(g, t.e0.j, k) + (size_t)r0 * M::stride(g);
+  t.e0.sl = l0 * p.q.lds_len + lds_slot(p.q, r0);
+  t.e1.r = r1;
+  t.e1.j = j0 + l1;
+  t.e1.ok = in && t.e1.j < M::nj(g) && r1 < n;
+  t.e1.gi = t.e0.gi + 1;
+  t.e1.sl = l1 * p.q.lds_len + lds_slot(p.q, r1);
+  return t;
+}
 
 // Stage one element of the tile's input field(s) in LDS.
 template <int DIR, bool HAS_S, int T>
@@ -684,6 +763,8 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   // the rows just before and after the segment
   const int sl_prev = sg.k == 0 ? 0 : base - q.stride + q.m - 1;
   const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
+  const LineSource<DIR> src(p.P, g, j, k);
+  (void)src;
   auto FA = [&](int i) { return la[lds_slot(q, i)]; };
   auto FS = [&](int i) { return lb[lds_slot(q, i)]; };
 #pragma unroll
@@ -711,8 +792,8 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
         } else {
           double v = tdt * sum + fc;
           if (RHS == kRhsTSZ) {
-            v = v + p.c_f1 * source(p.P, g, p.e_f1, out, ix, iy, iz);
-            v = v + p.c_f2 * source(p.P, g, p.e_f2, out, ix, iy, iz);
+            v = v + p.c_f1 * src.at(p.P, g, p.e_f1, out, j, k, i);
+            v = v + p.c_f2 * src.at(p.P, g, p.e_f2, out, j, k, i);
           }
           d[r] = v;
         }
@@ -759,6 +840,8 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, cons
   constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
   using M = LineMap<DIR>;
   const LinePart &q = p.q;
+  const LineSource<DIR> src(p.P, p.g, j, k);
+  (void)src;
   if (sg.m > 0) {
     const unsigned cm = EPI ? rows_outside(p.mib, lm, sg.s) : 0u;
     const int base = 1 + sg.k * q.stride;
@@ -766,11 +849,7 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, cons
     for (int r = 0; r < MMAX; r++) {
       if (r < sg.m) {
         double v = x[r];
-        if (EPI) {
-          int ix, iy, iz;
-          M::node(j, k, sg.s + r, ix, iy, iz);
-          v = v + p.c_f1 * source(p.P, p.g, p.e_f1, ((cm >> r) & 1u) != 0, ix, iy, iz);
-        }
+        if (EPI) v = v + p.c_f1 * src.at(p.P, p.g, p.e_f1, ((cm >> r) & 1u) != 0, j, k, sg.s + r);
         lo[base + r] = v;
       }
     }

@@ -253,28 +253,35 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   double *tA = smem;
   double *tB = smem + T * q.lds_len;
 
-  // stage the tile: kChunk loads in flight per thread, then their LDS stores
-  // (chunks one after the other, so the loads' registers stay bounded)
-  constexpr int kChunk = 6;
+  // stage the tile: 16-byte loads of element pairs, kChunk of them in flight
+  // per thread, then their LDS stores (chunks one after the other, so the
+  // loads' registers stay bounded)
+  constexpr int kChunk = 5;
+  const int nh = DIR == 0 ? (n + 1) >> 1 : 1;
+  const unsigned magic_nh = magic_of(nh);
 #pragma unroll 1
-  for (int u0 = 0; u0 < kTileElsPerThread; u0 += kChunk) {
-    double va[kChunk], vb[kChunk];
+  for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
+    double2 va[kChunk], vb[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
-      va[c] = 0.0;
-      vb[c] = 0.0;
-      if (t.ok) {
-        va[c] = p.A[t.gi];
-        if (HAS_S) vb[c] = p.S[t.gi];
+      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      va[c] = make_double2(0.0, 0.0);
+      vb[c] = make_double2(0.0, 0.0);
+      if (t.e0.ok) {
+        va[c] = *reinterpret_cast<const double2 *>(p.A + t.e0.gi);
+        if (HAS_S) vb[c] = *reinterpret_cast<const double2 *>(p.S + t.e0.gi);
       }
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
-      if (t.ok) {
-        tA[t.sl] = tile_in_value<DIR>(p, t, k, va[c]);
-        if (HAS_S) tB[t.sl] = vb[c];
+      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      if (t.e0.ok) {
+        tA[t.e0.sl] = tile_in_value<DIR>(p, t.e0, k, va[c].x);
+        if (HAS_S) tB[t.e0.sl] = vb[c].x;
+      }
+      if (t.e1.ok) {
+        tA[t.e1.sl] = tile_in_value<DIR>(p, t.e1, k, va[c].y);
+        if (HAS_S) tB[t.e1.sl] = vb[c].y;
       }
     }
   }
@@ -319,17 +326,25 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 
   const double *tO = TWO && !HAS_S ? tB : tA;
 #pragma unroll 1
-  for (int u0 = 0; u0 < kTileElsPerThread; u0 += kChunk) {
-    double vo[kChunk];
+  for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
+    double2 vo[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
-      vo[c] = t.ok ? tO[t.sl] : 0.0;
+      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      vo[c].x = t.e0.ok ? tO[t.e0.sl] : 0.0;
+      vo[c].y = t.e1.ok ? tO[t.e1.sl] : 0.0;
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TileAt t = tile_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_n);
-      if (t.ok && t.j >= 1 && t.j <= nj - 2) p.out[t.gi] = vo[c];
+      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
+      const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
+      if (w0 && w1) {
+        *reinterpret_cast<double2 *>(p.out + t.e0.gi) = vo[c];
+      } else {
+        if (w0) p.out[t.e0.gi] = vo[c].x;
+        if (w1) p.out[t.e1.gi] = vo[c].y;
+      }
     }
   }
 }
