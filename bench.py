@@ -42,6 +42,17 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain device copy of known size (the reference "
                          "kernel for scaling FETCH_SIZE / WRITE_SIZE counter readings)")
+    ap.add_argument("--workload", default="step", choices=["step", "batch", "serve"],
+                    help="step: the single-solve line above (default); batch: uneven "
+                         "independent solves, interleaved vs one after another; serve: "
+                         "Poisson request stream through the serving loop")
+    ap.add_argument("--solves", type=int, default=96, help="batch: number of solves")
+    ap.add_argument("--requests", type=int, default=2000, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=40.0, help="serve: arrivals per second")
+    ap.add_argument("--max-inflight", type=int, default=32, help="serve: in-flight limit")
+    ap.add_argument("--hbm-cap-gb", type=float, default=200.0, help="serve: admission cap")
+    ap.add_argument("--n-max", type=int, default=256, help="serve: largest N drawn")
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -51,6 +62,10 @@ def main():
 
     if args.copy_calibration:
         return copy_calibration()
+    if args.workload == "batch":
+        return bench_batch(args)
+    if args.workload == "serve":
+        return bench_serve(args)
 
     mts = load_mts()
     n = args.n + 1
@@ -153,6 +168,189 @@ def main():
             "avg_launch_us": avg_s * 1e6,
         },
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+METRIC = "grid-pt-steps/s at fixed L-inf error (FP32+batch vs FP64); solves/s + p50/p99"
+
+
+def draw_solve(rng, n, method, mib, T):
+    """One solve of the uneven mix: interface drawn from sphere r ~ U[0.3, 0.7],
+    ellipsoid semi-axes ~ U[0.3, 0.8] or torus R = 0.5, r ~ U[0.15, 0.25];
+    beta+ = 1 and beta- log-uniform in [1e-3, 1e3]; dt = h; t from 0 to T."""
+    kind = int(rng.integers(3))
+    if kind == 0:
+        surf = ("ellipsoid", (1.0, 1.0, 1.0, float(rng.uniform(0.3, 0.7))))
+    elif kind == 1:
+        surf = ("ellipsoid", tuple(float(v) for v in rng.uniform(0.3, 0.8, 3)) + (1.0,))
+    else:
+        surf = ("torus", (0.5, float(rng.uniform(0.15, 0.25))))
+    beta_minus = float(10.0 ** rng.uniform(-3.0, 3.0))
+    return {"n": n, "surface": surf, "beta": (beta_minus, 1.0), "method": method, "mib": mib,
+            "dt": 2.0 / n, "T": T}
+
+
+def describe(d):
+    return (f"{d['method']} MIB-L{d['mib']} N={d['n']} {d['surface'][0]}"
+            f"{tuple(round(v, 4) for v in d['surface'][1])} beta-={d['beta'][0]:.4g}")
+
+
+def bench_batch(args):
+    """Uneven independent solves (N in {32, 48, ..., 192}, Douglas-ADI or
+    LOD-CN, MIB-L1, T = 0.5, FP64): every solve's time loop run one after
+    another with mts_run, then all of them interleaved with mts_batch_run.
+    Setup and upload happen before either timed region."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy as np
+
+    mts = load_mts()
+    rng = np.random.default_rng(args.seed)
+    draws = [draw_solve(rng, int(rng.choice(np.arange(32, 193, 16))),
+                        ["adi", "lod-cn"][int(rng.integers(2))], 1, 0.5)
+             for _ in range(args.solves)]
+    ctx = mts.Context(0)
+
+    def make(d):
+        phys = mts.physics("const", *d["beta"])
+        try:
+            st = mts.Setup(mts.grid(d["n"] + 1), mts.surface(d["surface"][0], *d["surface"][1]),
+                           phys, mib=d["mib"])
+            return mts.Instance(ctx, st, d["method"], d["dt"]), None
+        except mts.MtsError as e:
+            return None, e.code
+
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(16) as ex:
+        made = list(ex.map(make, draws))
+    setup_s = time.perf_counter() - t0
+    refused = {}
+    insts, ds = [], []
+    for d, (inst, code) in zip(draws, made):
+        if inst is None:
+            refused[str(code)] = refused.get(str(code), 0) + 1
+        else:
+            insts.append(inst)
+            ds.append(d)
+    work = sum((d["n"] + 1) ** 3 * int(d["T"] / d["dt"]) for d in ds)
+
+    for inst in insts:
+        inst.initialize(0.0)
+        inst.sync()
+    # a loop of single solves: each one's time loop runs to completion before the next
+    t0 = time.perf_counter()
+    ends = []
+    for inst, d in zip(insts, ds):
+        ends.append(inst.run(0.0, d["T"])[1])
+        inst.sync()
+    loop_s = time.perf_counter() - t0
+    loop_err = [inst.error(t_end)[0] for inst, t_end in zip(insts, ends)]
+
+    for inst in insts:
+        inst.initialize(0.0)
+        inst.sync()
+    t0 = time.perf_counter()
+    out = mts.batch_run(insts, 0.0, [d["T"] for d in ds])
+    batch_s = time.perf_counter() - t0
+    same = True
+    for inst, (steps, t_end), le, te in zip(insts, out, loop_err, ends):
+        same &= t_end == te and inst.error(t_end)[0] == le
+    line = {
+        "metric": METRIC,
+        "value": work / batch_s,
+        "unit": "grid-pt-steps/s",
+        "n_gpus": 1,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.solves} uneven solves from seed {args.seed}: N in "
+                        "{32,48,...,192}, sphere/ellipsoid/torus, beta-/beta+ log-uniform "
+                        "in [1e-3,1e3], Douglas-ADI or LOD-CN, MIB-L1, dt=h, T=0.5",
+            "solves_run": len(insts),
+            "solves_refused": refused,
+            "grid_pt_steps": work,
+            "host_setup_s": setup_s,
+        },
+        "batched_s": batch_s,
+        "one_after_another_s": loop_s,
+        "one_after_another_value": work / loop_s,
+        "speedup": loop_s / batch_s,
+        "same_linf_batched_and_alone": bool(same),
+        "max_linf": max(loop_err) if loop_err else None,
+    }
+    for inst in insts:
+        inst.close()
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+def bench_serve(args):
+    """Poisson request stream through the serving loop: N uniform in
+    [24, n_max], T in {0.1, 0.25, 0.5, 1.0}, any method, MIB-L1 or L2, the
+    interface mix of the batch workload; admission capped by HBM bytes."""
+    import numpy as np
+
+    mts = load_mts()
+    rng = np.random.default_rng(args.seed)
+    draws = [draw_solve(rng, int(rng.integers(24, args.n_max + 1)),
+                        ["adi", "lod-ie", "lod-cn", "ts"][int(rng.integers(4))],
+                        int(rng.integers(1, 3)), float(rng.choice([0.1, 0.25, 0.5, 1.0])))
+             for _ in range(args.requests)]
+    gaps = rng.exponential(1.0 / args.rate, args.requests)
+    ctx = mts.Context(0)
+    srv = mts.Server(ctx, int(args.hbm_cap_gb * (1 << 30)), max_inflight=args.max_inflight,
+                     setup_threads=16)
+    t0 = time.perf_counter()
+    next_t = t0
+    for i, d in enumerate(draws):
+        next_t += gaps[i]
+        now = time.perf_counter()
+        if next_t > now:
+            time.sleep(next_t - now)
+        srv.submit(mts.request(mts.grid(d["n"] + 1),
+                               mts.surface(d["surface"][0], *d["surface"][1]),
+                               mts.physics("const", *d["beta"]), d["method"], d["dt"], d["T"],
+                               mib=d["mib"], tag=i))
+        if i % 200 == 199:
+            print(f"submitted {i + 1}", file=sys.stderr, flush=True)
+    results = srv.drain()
+    wall = time.perf_counter() - t0
+    srv.close()
+    ok = [r for r in results.values() if r["status"] == 0]
+    status = {}
+    for r in results.values():
+        status[str(r["status"])] = status.get(str(r["status"]), 0) + 1
+    lat = np.array([r["latency_ms"] for r in ok])
+    queue = np.array([r["queue_ms"] for r in ok])
+    work = sum((draws[r["tag"]]["n"] + 1) ** 3 * r["steps"] for r in ok)
+    worst = max(ok, key=lambda r: r["linf"]) if ok else None
+    line = {
+        "metric": METRIC,
+        "value": len(ok) / wall,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.requests} requests from seed {args.seed}, Poisson arrivals at "
+                        f"{args.rate}/s, N uniform in [24,{args.n_max}], T in "
+                        "{0.1,0.25,0.5,1.0}, ADI/LOD-IE/LOD-CN/TS, MIB-L1/L2, "
+                        "sphere/ellipsoid/torus, beta-/beta+ log-uniform in [1e-3,1e3], dt=h",
+            "hbm_cap_gb": args.hbm_cap_gb,
+            "max_inflight": args.max_inflight,
+        },
+        "status_counts": status,
+        "p50_latency_ms": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_latency_ms": float(np.percentile(lat, 99)) if len(lat) else None,
+        "p50_queue_ms": float(np.percentile(queue, 50)) if len(queue) else None,
+        "p99_queue_ms": float(np.percentile(queue, 99)) if len(queue) else None,
+        "grid_pt_steps_per_s": work / wall,
+        "wall_s": wall,
+        "max_linf": worst["linf"] if worst else None,
+        "max_linf_solve": describe(draws[worst["tag"]]) if worst else None,
+        "median_linf": float(np.median([r["linf"] for r in ok])) if ok else None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -167,6 +167,65 @@ int mts_profile_enable(mts_instance *inst, int on);
 int mts_profile_reset(mts_instance *inst);
 int mts_profile_read(mts_instance *inst, mts_stage_time *out, int capacity, int *count);
 
+/* The instance's time step. */
+int mts_instance_dt(const mts_instance *inst, double *dt);
+/* Device bytes the instance holds (field, MIB tables, stage records, scratch). */
+int mts_instance_device_bytes(const mts_instance *inst, size_t *bytes);
+
+/* ------------------------------------------------------------ batching
+ * Advance several instances over their time loops ((int)(t_finish[i] / dt)
+ * steps with t += dt from t0[i], as mts_run), issuing their steps
+ * interleaved so independent solves overlap on the device (each instance
+ * runs on its own stream).  No counterpart in the reference, which runs one
+ * solve per process; replaces a loop of mts_run calls. */
+int mts_batch_run(mts_instance *const *insts, int count, const double *t0, const double *t_finish,
+                  int *steps, double *t_end);
+
+/* ------------------------------------------------------------- serving
+ * A request loop with continuous batching: submitted solves get their host
+ * setup on setup threads, are admitted in arrival order while the sum of
+ * their device footprints stays under hbm_cap_bytes and at most
+ * max_inflight are in flight, and advance one step per round each, so
+ * solves that arrive while others run join between steps.  Fields and MIB
+ * tables stay on the device for a solve's lifetime.  No counterpart in the
+ * reference (one solve per process, helper.h:199-401 loops). */
+typedef struct {
+  mts_grid_desc grid;
+  mts_surface_desc surface;   /* CALLBACK surfaces are called from a setup thread */
+  mts_physics_desc physics;
+  int mib;                    /* 1 or 2 */
+  int method;                 /* MTS_ADI .. MTS_TS */
+  double dt, t0, t_finish;
+  long long tag;              /* returned in the result */
+} mts_request;
+
+typedef struct {
+  long long id, tag;
+  int status;                 /* MTS_OK or an MTS_E_* code */
+  char message[160];
+  int steps;
+  double t_end, linf, l2;     /* error norms against the exact solution at t_end */
+  int argmax[3];
+  double setup_ms;            /* host setup */
+  double queue_ms;            /* submit -> admitted to the device */
+  double run_ms;              /* admitted -> done */
+  double latency_ms;          /* submit -> done */
+  size_t footprint;           /* device bytes charged against the cap */
+} mts_result;
+
+typedef struct mts_server mts_server;
+
+int mts_server_create(mts_ctx *ctx, size_t hbm_cap_bytes, int max_inflight, int setup_threads,
+                      mts_server **out);
+/* Thread-safe; returns at once with the request id. */
+int mts_server_submit(mts_server *s, const mts_request *req, long long *id);
+/* Moves up to capacity finished results (in completion order) to out. */
+int mts_server_poll(mts_server *s, mts_result *out, int capacity, int *count);
+/* Blocks until at most max_outstanding submitted solves are unfinished. */
+int mts_server_wait(mts_server *s, long long max_outstanding);
+/* Stops the loop; solves not yet finished are dropped. */
+void mts_server_destroy(mts_server *s);
+
 /* Batched tridiagonal solve (the reference's TDMA, douglas_adi.cc:1784-1807,
  * for many lines at once): nlines systems of length n, host arrays line-major;
  * a[l*n] and c[l*n + n - 1] are ignored. */

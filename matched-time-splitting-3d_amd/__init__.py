@@ -77,6 +77,27 @@ class SetupInfo(C.Structure):
                 ("corner_pairs", C.c_int * 3), ("singular_warnings", C.c_int), ("mib", C.c_int)]
 
 
+class Request(C.Structure):
+    _fields_ = [("grid", GridDesc), ("surface", SurfaceDesc), ("physics", PhysicsDesc),
+                ("mib", C.c_int), ("method", C.c_int), ("dt", C.c_double), ("t0", C.c_double),
+                ("t_finish", C.c_double), ("tag", C.c_longlong)]
+
+
+class Result(C.Structure):
+    _fields_ = [("id", C.c_longlong), ("tag", C.c_longlong), ("status", C.c_int),
+                ("message", C.c_char * 160), ("steps", C.c_int), ("t_end", C.c_double),
+                ("linf", C.c_double), ("l2", C.c_double), ("argmax", C.c_int * 3),
+                ("setup_ms", C.c_double), ("queue_ms", C.c_double), ("run_ms", C.c_double),
+                ("latency_ms", C.c_double), ("footprint", C.c_size_t)]
+
+    def as_dict(self) -> dict:
+        return {"id": self.id, "tag": self.tag, "status": self.status,
+                "message": self.message.decode(), "steps": self.steps, "t_end": self.t_end,
+                "linf": self.linf, "l2": self.l2, "argmax": tuple(self.argmax),
+                "setup_ms": self.setup_ms, "queue_ms": self.queue_ms, "run_ms": self.run_ms,
+                "latency_ms": self.latency_ms, "footprint": self.footprint}
+
+
 _lib_handle = None
 
 
@@ -115,6 +136,14 @@ def lib() -> C.CDLL:
             "mts_profile_enable": ([VP, I], I),
             "mts_profile_reset": ([VP], I),
             "mts_profile_read": ([VP, VP, I, P(I)], I),
+            "mts_instance_dt": ([VP, P(D)], I),
+            "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
+            "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
+            "mts_server_create": ([VP, C.c_size_t, I, I, P(VP)], I),
+            "mts_server_submit": ([VP, P(Request), P(C.c_longlong)], I),
+            "mts_server_poll": ([VP, VP, I, P(I)], I),
+            "mts_server_wait": ([VP, C.c_longlong], I),
+            "mts_server_destroy": ([VP], None),
         }
         for name, (args, res) in sig.items():
             f = getattr(L, name)
@@ -333,6 +362,12 @@ class Instance:
         return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms)
                 for i in range(min(cnt.value, 64))}
 
+    def device_bytes(self) -> int:
+        """Device memory the instance holds."""
+        b = C.c_size_t()
+        _check(lib().mts_instance_device_bytes(self._h, C.byref(b)))
+        return b.value
+
     def device_field(self):
         p, pitch, st = C.c_void_p(), C.c_int(), C.c_void_p()
         _check(lib().mts_field_device(self._h, C.byref(p), C.byref(pitch), C.byref(st)))
@@ -341,3 +376,74 @@ class Instance:
 
 def version() -> str:
     return lib().mts_version().decode()
+
+
+def batch_run(instances, t0, t_finish):
+    """Runs the time loops of several instances with their steps interleaved
+    (``mts_batch_run``); returns [(steps, t_end)] per instance."""
+    n = len(instances)
+    t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, np.float64), (n,)))
+    tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_finish, np.float64), (n,)))
+    hs = (C.c_void_p * n)(*[i._h.value for i in instances])
+    steps = np.zeros(n, np.int32)
+    t_end = np.zeros(n, np.float64)
+    _check(lib().mts_batch_run(hs, n, _dptr(t0), _dptr(tf), _dptr(steps), _dptr(t_end)))
+    return [(int(steps[i]), float(t_end[i])) for i in range(n)]
+
+
+def request(g: GridDesc, surf: SurfaceDesc, phys: PhysicsDesc, method, dt: float,
+            t_finish: float, mib: int = 1, t0: float = 0.0, tag: int = 0) -> Request:
+    """An ``mts_request`` for :meth:`Server.submit`."""
+    m = METHODS[method] if isinstance(method, str) else int(method)
+    return Request(g, surf, phys, mib, m, dt, t0, t_finish, tag)
+
+
+class Server:
+    """The continuous-batching request loop (``mts_server_*``): solves are set
+    up on host threads, admitted under an HBM cap and an in-flight limit, and
+    advanced one step per round each on their own streams."""
+
+    def __init__(self, ctx: Context, hbm_cap_bytes: int, max_inflight: int = 8,
+                 setup_threads: int = 4):
+        self._h = C.c_void_p()
+        self.ctx = ctx
+        self._keep = []  # requests stay referenced while the loop may read them
+        _check(lib().mts_server_create(ctx.handle, int(hbm_cap_bytes), int(max_inflight),
+                                       int(setup_threads), C.byref(self._h)))
+
+    def submit(self, req: Request) -> int:
+        rid = C.c_longlong()
+        self._keep.append(req)
+        _check(lib().mts_server_submit(self._h, C.byref(req), C.byref(rid)))
+        return rid.value
+
+    def poll(self, capacity: int = 256):
+        buf = (Result * capacity)()
+        cnt = C.c_int()
+        _check(lib().mts_server_poll(self._h, buf, capacity, C.byref(cnt)))
+        return [buf[i].as_dict() for i in range(cnt.value)]
+
+    def wait(self, max_outstanding: int = 0):
+        _check(lib().mts_server_wait(self._h, int(max_outstanding)))
+
+    def drain(self):
+        """Waits for every submitted solve and returns all results by id."""
+        self.wait(0)
+        out = {}
+        while True:
+            got = self.poll()
+            if not got:
+                return out
+            for r in got:
+                out[r["id"]] = r
+
+    def close(self):
+        if self._h:
+            lib().mts_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

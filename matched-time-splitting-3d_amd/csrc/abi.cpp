@@ -13,6 +13,7 @@
 #include <string>
 #include <vector>
 
+#include "abi_internal.hpp"
 #include "device_core.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
@@ -27,6 +28,12 @@ int fail(int code, const std::string &msg) {
   g_err = msg;
   return code;
 }
+
+}  // namespace
+
+int mts::set_error(int code, const char *msg) { return fail(code, msg ? msg : ""); }
+
+namespace {
 
 #define MTS_HIP(call)                                                                      \
   do {                                                                                     \
@@ -67,24 +74,44 @@ bool make_beta(const mts_physics_desc *pd, mts::BetaModel &b, std::string &err) 
   return true;
 }
 
+// Device buffer.  With a stream it is allocated and freed stream-ordered
+// (hipMallocAsync / hipFreeAsync from the device's memory pool), so creating
+// and destroying instances does not synchronize the device while other
+// instances' steps are in flight; the stream must outlive the buffer.
 template <class T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  hipStream_t st = nullptr;
   ~DevBuf() {
-    if (p) (void)hipFree(p);
+    if (p) (void)(st ? hipFreeAsync(p, st) : hipFree(p));
   }
-  hipError_t alloc(size_t count) {
+  hipError_t alloc(size_t count, hipStream_t s = nullptr) {
     n = count;
+    st = s;
     if (count == 0) return hipSuccess;
+    if (s) return hipMallocAsync(reinterpret_cast<void **>(&p), count * sizeof(T), s);
     return hipMalloc(reinterpret_cast<void **>(&p), count * sizeof(T));
   }
-  // Allocates and copies on stream st.  The copy is asynchronous: h must stay
-  // valid, and no other stream may read the buffer, until st is synchronized.
-  hipError_t upload(const T *h, size_t count, hipStream_t st) {
-    hipError_t e = alloc(count);
+  // Allocates and copies on stream s.  The copy is asynchronous: h must stay
+  // valid, and no other stream may read the buffer, until s is synchronized.
+  size_t bytes() const { return p ? n * sizeof(T) : 0; }
+  hipError_t upload(const T *h, size_t count, hipStream_t s) {
+    hipError_t e = alloc(count, s);
     if (e != hipSuccess || count == 0) return e;
-    return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, st);
+    return hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s);
+  }
+};
+
+// Owns a stream; declared before an instance's buffers so that it is
+// destroyed after them (their stream-ordered frees need it).
+struct StreamHolder {
+  hipStream_t s = nullptr;
+  ~StreamHolder() {
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
   }
 };
 
@@ -108,7 +135,8 @@ struct mts_instance {
   mts::Grid grid;
   int px = 0;
   size_t field_elems = 0;
-  hipStream_t stream = nullptr;
+  StreamHolder sh;                // the instance's stream (outlives the buffers below)
+  hipStream_t stream = nullptr;   // == sh.s
   DevBuf<double> u, r;            // solution; Douglas stage scratch
   DevBuf<int8_t> ind;
   DevBuf<double> tab[3];
@@ -142,11 +170,17 @@ struct mts_instance {
   std::vector<double> stage_ms;
   std::vector<int> stage_n;
   ~mts_instance() {
-    if (stream) {
-      (void)hipStreamSynchronize(stream);  // pending copies may read host staging
-      (void)hipStreamDestroy(stream);
-    }
+    if (stream) (void)hipStreamSynchronize(stream);  // pending copies may read host staging
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
+  size_t device_bytes() const {
+    size_t b = u.bytes() + r.bytes() + ind.bytes() + spec.bytes() + part_max.bytes() +
+               part_sq.bytes() + part_idx.bytes();
+    for (int d = 0; d < 3; d++)
+      b += tab[d].bytes() + blk_ptr[d].bytes() + blk_hdr[d].bytes() + blk_val[d].bytes() +
+           cr_val[d].bytes() + seq[d].bytes() + cls0[d].bytes();
+    for (auto &kv : recs) b += kv.second->bytes();
+    return b;
   }
   int stage(const char *name) {
     auto it = stage_id.find(name);
@@ -206,6 +240,195 @@ hipError_t timed(mts_instance *in, const char *name, F launch) {
   return hipSuccess;
 }
 }  // namespace
+
+// Host half of instance creation: everything but device memory, so that a
+// serving loop can build it on a setup thread and know the exact device
+// footprint before admitting the solve.
+struct mts::InstancePlan {
+  int method = 0;
+  int min_lanes = 16;
+  double dt = 0.0, theta_dt = 0.0;
+  mts::Grid grid;
+  int px = 0;
+  size_t field_elems = 0;
+  mts::DeviceTableHost t;
+  std::vector<int8_t> ind;                              // x-padded node classification
+  std::vector<std::pair<int, std::vector<double>>> recs;  // (record key, stage records)
+  size_t device_bytes = 0;
+};
+
+void mts::PlanDeleter::operator()(mts::InstancePlan *p) const { delete p; }
+
+size_t mts::plan_device_bytes(const mts::InstancePlan &p) { return p.device_bytes; }
+
+int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, int method,
+                       double dt, mts::PlanPtr &out) {
+  if (!setup || !physics) return fail(MTS_E_ARG, "null argument");
+  if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
+  if (!(dt > 0.0)) return fail(MTS_E_ARG, "dt must be positive");
+  mts::BetaModel beta;
+  std::string err;
+  if (!make_beta(physics, beta, err)) return fail(MTS_E_ARG, err);
+  if (beta.kind != setup->beta.kind || beta.p[0] != setup->beta.p[0] ||
+      beta.p[1] != setup->beta.p[1])
+    return fail(MTS_E_ARG, "physics differs from the one the setup was built with");
+  if (method == MTS_ADI) {
+    const int *cp = setup->res.corner_pairs;
+    if (cp[0] + cp[1] + cp[2] > 0)
+      return fail(MTS_E_DISABLED,
+                  "Douglas-ADI on an interface with corner pairs is switched off on the GPU: "
+                  "its step raised a GPU memory aperture violation (dispatch reported as "
+                  "spec_kernel) whose cause is not found; Douglas-ADI without corner pairs, "
+                  "and LOD-IE, LOD-CN and TS with or without them, are not affected");
+  }
+  try {
+    mts::PlanPtr p(new mts::InstancePlan());
+    p->method = method;
+    p->dt = dt;
+    p->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
+    p->grid = setup->res.grid;
+    const mts::Grid &g = p->grid;
+    // MTS_LINE_LANES=32|64 raises the lane-group lower bound of the line split
+    // (a tuning knob: every split computes the same solution to rounding)
+    if (const char *ev = std::getenv("MTS_LINE_LANES")) {
+      const int v = std::atoi(ev);
+      if (v == 16 || v == 32 || v == 64) p->min_lanes = v;
+    }
+    for (int d = 0; d < 3; d++) {
+      mts::LinePart q;
+      if (!mts::make_line_part(g.n[d], q, p->min_lanes))
+        return fail(MTS_E_ARG, "line length above 1024 nodes is not supported");
+    }
+    p->px = (g.n[0] + 15) / 16 * 16;
+    p->field_elems = (size_t)p->px * g.n[1] * g.n[2];
+    mts::DeviceTableHost &t = p->t;
+    const std::string terr =
+        mts::build_device_table(setup->res, beta, p->theta_dt, method == MTS_ADI, t);
+    if (!terr.empty()) return fail(MTS_E_GEOMETRY, terr);
+    p->ind.assign(p->field_elems, 1);
+    for (int iz = 0; iz < g.n[2]; iz++)
+      for (int iy = 0; iy < g.n[1]; iy++)
+        std::memcpy(&p->ind[((size_t)iz * g.n[1] + iy) * p->px],
+                    &setup->res.ind[g.idx(0, iy, iz)], g.n[0]);
+    using namespace mts;
+    std::vector<std::pair<int, int>> kinds;  // (direction, stage kind) this method launches
+    for (int d = 0; d < 3; d++) {
+      if (method == MTS_ADI) kinds.push_back({d, d == 0 ? kRhsCopy | kFlagSolve : kRhsDouglas | kFlagSolve});
+      else if (method == MTS_LOD_IE) kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
+      else if (method == MTS_LOD_CN) kinds.push_back({d, kRhsCN | kFlagSolve | kFlagLhsJump});
+      else if (d < 2) {
+        kinds.push_back({d, kRhsCN});
+        kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
+      } else {
+        kinds.push_back({d, kRhsTSZ | kFlagSolve | kFlagLhsJump});
+      }
+    }
+    for (auto &kd : kinds) {
+      std::vector<double> rec;
+      const std::string rerr = build_stage_records(setup->res, t, kd.first, kd.second,
+                                                   p->theta_dt, dt, 0.5 * dt, 0.5 * dt, rec);
+      if (!rerr.empty()) return fail(MTS_E_ARG, rerr);
+      p->recs.push_back({mts_instance::rec_key(kd.first, kd.second), std::move(rec)});
+    }
+    // exactly what instance_from_plan allocates
+    size_t b = p->field_elems * (sizeof(double) * (method == MTS_ADI ? 2 : 1) + sizeof(int8_t));
+    for (int d = 0; d < 3; d++)
+      b += t.tab[d].size() * sizeof(double) + t.blk_ptr[d].size() * sizeof(int) +
+           t.blk_hdr[d].size() * sizeof(mts::BlockHdr) + t.blk_val[d].size() * sizeof(double) +
+           t.cr_val[d].size() * sizeof(double) + (t.n_seq[d] > 0 ? t.seq[d].size() : 0) +
+           t.cls0[d].size();
+    for (auto &r : p->recs) b += r.second.size() * sizeof(double);
+    b += t.spec.size() * sizeof(int);
+    mts::DevGeom dg;
+    for (int a = 0; a < 3; a++) dg.n[a] = g.n[a];
+    b += (size_t)mts::norms_blocks(dg) * (2 * sizeof(double) + sizeof(long long));
+    p->device_bytes = b;
+    out = std::move(p);
+  } catch (const std::bad_alloc &) {
+    return fail(MTS_E_NOMEM, "out of host memory planning the instance");
+  }
+  return MTS_OK;
+}
+
+int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out) {
+  if (!ctx || !plan || !out) return fail(MTS_E_ARG, "null argument");
+  *out = nullptr;
+  mts::InstancePlan &p = *plan;
+  const mts::Grid &g = p.grid;
+  for (int d = 0; d < 3; d++) {
+    mts::LinePart q;
+    mts::make_line_part(g.n[d], q, p.min_lanes);
+    const size_t need = (size_t)mts::tile_lines(q.L) * q.lds_len * sizeof(double) * 2;
+    if (need > ctx->prop.sharedMemPerBlock)
+      return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
+  }
+  MTS_HIP(hipSetDevice(ctx->device));
+  std::unique_ptr<mts_instance> in(new mts_instance());
+  in->ctx = ctx;
+  in->method = p.method;
+  in->dt = p.dt;
+  in->theta_dt = p.theta_dt;
+  in->grid = g;
+  in->min_lanes = p.min_lanes;
+  in->px = p.px;
+  in->field_elems = p.field_elems;
+  MTS_HIP(hipStreamCreateWithFlags(&in->sh.s, hipStreamNonBlocking));
+  in->stream = in->sh.s;
+  const mts::DeviceTableHost &t = p.t;
+  in->phys = t.phys;
+
+  // Every fill and copy below goes on the instance's own stream, which is
+  // non-blocking: work on the legacy null stream (hipMemset, and the staged
+  // DMA of a pageable hipMemcpy, which may still be in flight when hipMemcpy
+  // returns) is not ordered before the kernels launched on it.  The plan's
+  // host staging stays alive until the stream is synchronized at the end.
+  const hipStream_t st = in->stream;
+  MTS_HIP(in->u.alloc(in->field_elems, st));
+  MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
+  if (p.method == MTS_ADI) {
+    MTS_HIP(in->r.alloc(in->field_elems, st));
+    MTS_HIP(hipMemsetAsync(in->r.p, 0, in->field_elems * sizeof(double), st));
+  }
+  MTS_HIP(in->ind.upload(p.ind.data(), p.ind.size(), st));
+  for (int a = 0; a < 3; a++) MTS_HIP(in->tab[a].upload(t.tab[a].data(), t.tab[a].size(), st));
+  for (int d = 0; d < 3; d++) {
+    MTS_HIP(in->blk_ptr[d].upload(t.blk_ptr[d].data(), t.blk_ptr[d].size(), st));
+    MTS_HIP(in->blk_hdr[d].upload(t.blk_hdr[d].data(), t.blk_hdr[d].size(), st));
+    MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size(), st));
+    MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size(), st));
+    if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
+    MTS_HIP(in->cls0[d].upload(t.cls0[d].data(), t.cls0[d].size(), st));
+  }
+  for (auto &r : p.recs) {
+    std::unique_ptr<DevBuf<double>> b(new DevBuf<double>());
+    MTS_HIP(b->upload(r.second.data(), r.second.size(), st));
+    in->recs[r.first] = std::move(b);
+  }
+  in->nspec = (int)(t.spec.size() / mts::kSpecLen);
+  MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size(), st));
+  mts::DevGeom &dg = in->geom;
+  for (int a = 0; a < 3; a++) {
+    dg.n[a] = g.n[a];
+    dg.tab[a] = in->tab[a].p;
+  }
+  dg.px = in->px;
+  dg.ind = in->ind.p;
+  for (int d = 0; d < 3; d++) {
+    in->mib[d].blk_ptr = in->blk_ptr[d].p;
+    in->mib[d].hdr = in->blk_hdr[d].p;
+    in->mib[d].val = in->blk_val[d].p;
+    in->mib[d].cr = in->cr_val[d].p;
+    in->mib[d].seq = in->seq[d].p;
+    in->mib[d].cls0 = in->cls0[d].p;
+  }
+  int nb = mts::norms_blocks(dg);
+  MTS_HIP(in->part_max.alloc(nb, st));
+  MTS_HIP(in->part_sq.alloc(nb, st));
+  MTS_HIP(in->part_idx.alloc(nb, st));
+  MTS_HIP(hipStreamSynchronize(st));
+  *out = in.release();
+  return MTS_OK;
+}
 
 extern "C" {
 
@@ -327,143 +550,12 @@ int mts_ctx_device_info(mts_ctx *ctx, char *name, size_t name_len, size_t *hbm_b
 
 int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
                         int method, double dt, mts_instance **out) {
-  if (!ctx || !setup || !physics || !out) return fail(MTS_E_ARG, "null argument");
+  if (!ctx || !out) return fail(MTS_E_ARG, "null argument");
   *out = nullptr;
-  if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
-  if (!(dt > 0.0)) return fail(MTS_E_ARG, "dt must be positive");
-  mts::BetaModel beta;
-  std::string err;
-  if (!make_beta(physics, beta, err)) return fail(MTS_E_ARG, err);
-  if (beta.kind != setup->beta.kind || beta.p[0] != setup->beta.p[0] ||
-      beta.p[1] != setup->beta.p[1])
-    return fail(MTS_E_ARG, "physics differs from the one the setup was built with");
-  if (method == MTS_ADI) {
-    const int *cp = setup->res.corner_pairs;
-    if (cp[0] + cp[1] + cp[2] > 0)
-      return fail(MTS_E_DISABLED,
-                  "Douglas-ADI on an interface with corner pairs is switched off on the GPU: "
-                  "its step raised a GPU memory aperture violation (dispatch reported as "
-                  "spec_kernel) whose cause is not found; Douglas-ADI without corner pairs, "
-                  "and LOD-IE, LOD-CN and TS with or without them, are not affected");
-  }
-  MTS_HIP(hipSetDevice(ctx->device));
-  std::unique_ptr<mts_instance> in(new mts_instance());
-  in->ctx = ctx;
-  in->method = method;
-  in->dt = dt;
-  in->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
-  in->grid = setup->res.grid;
-  const mts::Grid &g = in->grid;
-  // MTS_LINE_LANES=32|64 raises the lane-group lower bound of the line split
-  // (a tuning knob: every split computes the same solution to rounding)
-  if (const char *ev = std::getenv("MTS_LINE_LANES")) {
-    const int v = std::atoi(ev);
-    if (v == 16 || v == 32 || v == 64) in->min_lanes = v;
-  }
-  for (int d = 0; d < 3; d++) {
-    mts::LinePart q;
-    if (!mts::make_line_part(g.n[d], q, in->min_lanes))
-      return fail(MTS_E_ARG, "line length above 1024 nodes is not supported");
-    const size_t need = (size_t)mts::tile_lines(q.L) * q.lds_len * sizeof(double) * 2;
-    if (need > ctx->prop.sharedMemPerBlock)
-      return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
-  }
-  in->px = (g.n[0] + 15) / 16 * 16;
-  in->field_elems = (size_t)in->px * g.n[1] * g.n[2];
-  MTS_HIP(hipStreamCreateWithFlags(&in->stream, hipStreamNonBlocking));
-
-  mts::DeviceTableHost t;
-  try {
-    const std::string terr =
-        mts::build_device_table(setup->res, beta, in->theta_dt, method == MTS_ADI, t);
-    if (!terr.empty()) return fail(MTS_E_GEOMETRY, terr);
-  } catch (const std::bad_alloc &) {
-    return fail(MTS_E_NOMEM, "out of host memory building the MIB table");
-  }
-  in->phys = t.phys;
-
-  // Every fill and copy below goes on the instance's own stream, which is
-  // non-blocking: work on the legacy null stream (hipMemset, and the staged
-  // DMA of a pageable hipMemcpy, which may still be in flight when hipMemcpy
-  // returns) is not ordered before the kernels launched on it.  The host
-  // staging (t, ind) stays alive until the stream is synchronized at the end.
-  const hipStream_t st = in->stream;
-  MTS_HIP(in->u.alloc(in->field_elems));
-  MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
-  if (method == MTS_ADI) {
-    MTS_HIP(in->r.alloc(in->field_elems));
-    MTS_HIP(hipMemsetAsync(in->r.p, 0, in->field_elems * sizeof(double), st));
-  }
-  std::vector<int8_t> ind(in->field_elems, 1);
-  for (int iz = 0; iz < g.n[2]; iz++)
-    for (int iy = 0; iy < g.n[1]; iy++)
-      std::memcpy(&ind[((size_t)iz * g.n[1] + iy) * in->px], &setup->res.ind[g.idx(0, iy, iz)],
-                  g.n[0]);
-  MTS_HIP(in->ind.upload(ind.data(), ind.size(), st));
-  for (int a = 0; a < 3; a++) MTS_HIP(in->tab[a].upload(t.tab[a].data(), t.tab[a].size(), st));
-  for (int d = 0; d < 3; d++) {
-    MTS_HIP(in->blk_ptr[d].upload(t.blk_ptr[d].data(), t.blk_ptr[d].size(), st));
-    MTS_HIP(in->blk_hdr[d].upload(t.blk_hdr[d].data(), t.blk_hdr[d].size(), st));
-    MTS_HIP(in->blk_val[d].upload(t.blk_val[d].data(), t.blk_val[d].size(), st));
-    MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size(), st));
-    if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
-    MTS_HIP(in->cls0[d].upload(t.cls0[d].data(), t.cls0[d].size(), st));
-  }
-  {
-    using namespace mts;
-    std::vector<std::pair<int, int>> kinds;  // (direction, stage kind) this method launches
-    for (int d = 0; d < 3; d++) {
-      if (method == MTS_ADI) kinds.push_back({d, d == 0 ? kRhsCopy | kFlagSolve : kRhsDouglas | kFlagSolve});
-      else if (method == MTS_LOD_IE) kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
-      else if (method == MTS_LOD_CN) kinds.push_back({d, kRhsCN | kFlagSolve | kFlagLhsJump});
-      else if (d < 2) {
-        kinds.push_back({d, kRhsCN});
-        kinds.push_back({d, kRhsCopy | kFlagSolve | kFlagLhsJump});
-      } else {
-        kinds.push_back({d, kRhsTSZ | kFlagSolve | kFlagLhsJump});
-      }
-    }
-    for (auto &kd : kinds) {
-      std::vector<double> rec;
-      std::string rerr;
-      try {
-        rerr = build_stage_records(setup->res, t, kd.first, kd.second, in->theta_dt, dt, 0.5 * dt,
-                                   0.5 * dt, rec);
-      } catch (const std::bad_alloc &) {
-        return fail(MTS_E_NOMEM, "out of host memory building the stage records");
-      }
-      if (!rerr.empty()) return fail(MTS_E_ARG, rerr);
-      std::unique_ptr<DevBuf<double>> b(new DevBuf<double>());
-      MTS_HIP(b->upload(rec.data(), rec.size(), st));
-      // the host vector must outlive the asynchronous copy
-      MTS_HIP(hipStreamSynchronize(st));
-      in->recs[mts_instance::rec_key(kd.first, kd.second)] = std::move(b);
-    }
-  }
-  in->nspec = (int)(t.spec.size() / mts::kSpecLen);
-  MTS_HIP(in->spec.upload(t.spec.data(), t.spec.size(), st));
-  mts::DevGeom &dg = in->geom;
-  for (int a = 0; a < 3; a++) {
-    dg.n[a] = g.n[a];
-    dg.tab[a] = in->tab[a].p;
-  }
-  dg.px = in->px;
-  dg.ind = in->ind.p;
-  for (int d = 0; d < 3; d++) {
-    in->mib[d].blk_ptr = in->blk_ptr[d].p;
-    in->mib[d].hdr = in->blk_hdr[d].p;
-    in->mib[d].val = in->blk_val[d].p;
-    in->mib[d].cr = in->cr_val[d].p;
-    in->mib[d].seq = in->seq[d].p;
-    in->mib[d].cls0 = in->cls0[d].p;
-  }
-  int nb = mts::norms_blocks(dg);
-  MTS_HIP(in->part_max.alloc(nb));
-  MTS_HIP(in->part_sq.alloc(nb));
-  MTS_HIP(in->part_idx.alloc(nb));
-  MTS_HIP(hipStreamSynchronize(st));
-  *out = in.release();
-  return MTS_OK;
+  mts::PlanPtr plan;
+  int rc = mts::plan_instance(setup, physics, method, dt, plan);
+  if (rc) return rc;
+  return mts::instance_from_plan(ctx, std::move(plan), out);
 }
 
 void mts_instance_destroy(mts_instance *inst) {
@@ -705,6 +797,18 @@ int mts_error_norms(mts_instance *in, double t, double *linf, double *l2, int ar
     argmax[1] = (int)((mi / nz) % ny);
     argmax[2] = (int)(mi % nz);
   }
+  return MTS_OK;
+}
+
+int mts_instance_dt(const mts_instance *in, double *dt) {
+  if (!in || !dt) return fail(MTS_E_ARG, "null argument");
+  *dt = in->dt;
+  return MTS_OK;
+}
+
+int mts_instance_device_bytes(const mts_instance *in, size_t *bytes) {
+  if (!in || !bytes) return fail(MTS_E_ARG, "null argument");
+  *bytes = in->device_bytes();
   return MTS_OK;
 }
 

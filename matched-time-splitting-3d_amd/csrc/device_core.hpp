@@ -67,6 +67,7 @@ MTS_HD double source(const PhysConst &P, const DevGeom &g, double e,
 // Operator weights beta(x -+ h/2)/h^2 along axis d on the node's side.
 MTS_HD void op_weights(const PhysConst &P, const DevGeom &g, int d,
                                            bool outside, int ix, int iy, int iz, double vd[3]) {
+  (void)iz;  // beta depends on x and y only
   if (outside || P.beta_kind == 0) {
     const int sd = outside ? 1 : 0;
     vd[0] = P.vdc[d][sd][0];

@@ -38,6 +38,36 @@ def test_null_arguments_are_errors_not_crashes(mts):
     assert b"null" in L.mts_last_error()
     assert L.mts_step(None, 0.1) == -1
     assert L.mts_ctx_create(0, None) == -1
+    assert L.mts_batch_run(None, 1, None, None, None, None) == -1
+    assert L.mts_batch_run(None, 0, None, None, None, None) == 0  # nothing to run
+    assert L.mts_server_create(None, 1 << 30, 4, 2, None) == -1
+    assert L.mts_server_submit(None, None, None) == -1
+    assert L.mts_server_wait(None, 0) == -1
+    L.mts_server_destroy(None)
+
+
+def test_request_and_result_layout_match_the_header(mts, tmp_path):
+    """ctypes mirrors of mts_request / mts_result agree with the C compiler's layout."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        import pytest
+        pytest.skip("no C compiler")
+    src = tmp_path / "layout.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "mts.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mts_request), '
+        'offsetof(mts_request, tag), sizeof(mts_result), offsetof(mts_result, message), '
+        'offsetof(mts_result, argmax), offsetof(mts_result, footprint));return 0;}\n')
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
+                                          text=True).stdout.split()]
+    want = [ctypes.sizeof(mts.Request), mts.Request.tag.offset, ctypes.sizeof(mts.Result),
+            mts.Result.message.offset, mts.Result.argmax.offset, mts.Result.footprint.offset]
+    assert got == want
 
 
 def test_oracle_library_exports(oracle):
