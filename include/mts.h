@@ -111,6 +111,30 @@ int mts_setup_coords(const mts_setup *s, int axis, double *out, size_t count);
 /* crossings of direction dir in line order (Intersections::ifpx/ifpy/ifpz) */
 int mts_setup_crossings(const mts_setup *s, int dir, mts_crossing *out, size_t count);
 
+/* Node classification alone, the Mesh constructor (mesh/mesh.cc:20-70):
+ * +1 outside, -1 inside, MTS_LAYOUT_XFAST; MTS_E_GEOMETRY when the interface
+ * passes through a node (where the reference exits). */
+int mts_mesh_classify(const mts_grid_desc *grid, const mts_surface_desc *surface, int8_t *out,
+                      size_t count);
+
+/* ------------------------------------------------------------ dense LU
+ * The reference's LU class (decomposition/lu.h:7-21, lu.cc:15-196): Crout
+ * decomposition with implicit-scaled partial pivoting (Numerical Recipes
+ * ludcmp/lubksb), a zero pivot replaced by 1e-40.  Matrices are n*n
+ * row-major; a matrix with a zero row is MTS_E_ARG ("Singular matrix in
+ * LUdcmp", where the reference exits). */
+typedef struct mts_lu mts_lu;
+int mts_lu_create(int n, const double *a, mts_lu **out);     /* LU::LU (lu.cc:15-79) */
+void mts_lu_destroy(mts_lu *lu);
+int mts_lu_solve(const mts_lu *lu, const double *b, double *x); /* LU::solve(Vec) (lu.cc:82-121); b may alias x */
+/* LU::solve(Matrix) (lu.cc:124-143): b and x are n*m row-major, solved column by column */
+int mts_lu_solve_matrix(const mts_lu *lu, int m, const double *b, double *x);
+int mts_lu_inverse(const mts_lu *lu, double *ainv);            /* LU::inverse (lu.cc:146-162) */
+int mts_lu_det(const mts_lu *lu, double *det);                 /* LU::det (lu.cc:165-174) */
+/* LU::mprove (lu.cc:179-196): one step of iterative refinement of x for A x = b,
+ * the residual accumulated in long double */
+int mts_lu_mprove(const mts_lu *lu, const double *b, double *x);
+
 /* ---------------------------------------------------------------- device */
 int mts_ctx_create(int device, mts_ctx **out);
 void mts_ctx_destroy(mts_ctx *ctx);

@@ -122,6 +122,12 @@ struct mts_setup {
   mts::BetaModel beta;
 };
 
+struct mts_lu {
+  std::vector<double> a;  // the decomposed matrix, kept for mts_lu_mprove
+  mts::DenseLU lu;
+  mts_lu(int n, const double *m) : a(m, m + (size_t)n * n), lu(n, m) {}
+};
+
 struct mts_ctx {
   int device = 0;
   hipDeviceProp_t prop;
@@ -464,6 +470,88 @@ int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
 }
 
 void mts_setup_destroy(mts_setup *s) { delete s; }
+
+int mts_mesh_classify(const mts_grid_desc *grid, const mts_surface_desc *surface, int8_t *out,
+                      size_t count) {
+  if (!grid || !surface || !out) return fail(MTS_E_ARG, "null argument");
+  for (int a = 0; a < 3; a++) {
+    if (grid->n[a] < 2) return fail(MTS_E_ARG, "each axis needs at least 2 nodes");
+    if (!(grid->hi[a] > grid->lo[a])) return fail(MTS_E_ARG, "empty domain");
+  }
+  if (count < (size_t)grid->n[0] * grid->n[1] * grid->n[2])
+    return fail(MTS_E_ARG, "output smaller than the node count");
+  std::string err;
+  std::unique_ptr<mts::Surface> surf = make_surface(surface, err);
+  if (!surf) return fail(MTS_E_ARG, err);
+  const mts::Grid g = mts::make_grid(grid->lo, grid->hi, grid->n);
+  std::vector<int8_t> ind;
+  try {
+    err = mts::classify_nodes(g, *surf, ind);
+  } catch (const std::bad_alloc &) {
+    return fail(MTS_E_NOMEM, "out of host memory");
+  }
+  if (!err.empty()) return fail(MTS_E_GEOMETRY, err);
+  std::memcpy(out, ind.data(), ind.size());
+  return MTS_OK;
+}
+
+int mts_lu_create(int n, const double *a, mts_lu **out) {
+  if (n < 1 || !a || !out) return fail(MTS_E_ARG, "null argument or n < 1");
+  *out = nullptr;
+  std::unique_ptr<mts_lu> l(new (std::nothrow) mts_lu(n, a));
+  if (!l) return fail(MTS_E_NOMEM, "out of host memory");
+  // lu.cc:32-35: a row of zeros has no scale
+  if (l->lu.singular_row()) return fail(MTS_E_ARG, "Singular matrix in LUdcmp");
+  *out = l.release();
+  return MTS_OK;
+}
+
+void mts_lu_destroy(mts_lu *l) { delete l; }
+
+int mts_lu_solve(const mts_lu *l, const double *b, double *x) {
+  if (!l || !b || !x) return fail(MTS_E_ARG, "null argument");
+  l->lu.solve(b, x);
+  return MTS_OK;
+}
+
+int mts_lu_solve_matrix(const mts_lu *l, int m, const double *b, double *x) {
+  if (!l || !b || !x || m < 1) return fail(MTS_E_ARG, "null argument or m < 1");
+  const int n = l->lu.n();
+  std::vector<double> col(n);
+  for (int j = 0; j < m; j++) {  // column by column (lu.cc:124-143)
+    for (int i = 0; i < n; i++) col[i] = b[(size_t)i * m + j];
+    l->lu.solve(col.data(), col.data());
+    for (int i = 0; i < n; i++) x[(size_t)i * m + j] = col[i];
+  }
+  return MTS_OK;
+}
+
+int mts_lu_inverse(const mts_lu *l, double *ainv) {
+  if (!l || !ainv) return fail(MTS_E_ARG, "null argument");
+  l->lu.inverse(ainv);
+  return MTS_OK;
+}
+
+int mts_lu_det(const mts_lu *l, double *det) {
+  if (!l || !det) return fail(MTS_E_ARG, "null argument");
+  *det = l->lu.det();
+  return MTS_OK;
+}
+
+int mts_lu_mprove(const mts_lu *l, const double *b, double *x) {
+  if (!l || !b || !x) return fail(MTS_E_ARG, "null argument");
+  const int n = l->lu.n();
+  // residual A x - b accumulated in extended precision (lu.cc:179-196)
+  std::vector<double> r(n);
+  for (int i = 0; i < n; i++) {
+    long double sdp = -(long double)b[i];
+    for (int j = 0; j < n; j++) sdp += (long double)l->a[(size_t)i * n + j] * (long double)x[j];
+    r[i] = (double)sdp;
+  }
+  l->lu.solve(r.data(), r.data());
+  for (int i = 0; i < n; i++) x[i] -= r[i];
+  return MTS_OK;
+}
 
 int mts_setup_get_info(const mts_setup *s, mts_setup_info *info) {
   if (!s || !info) return fail(MTS_E_ARG, "null argument");

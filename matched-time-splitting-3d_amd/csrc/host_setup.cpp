@@ -675,18 +675,9 @@ struct Builder {
 
 }  // namespace
 
-std::string build_setup(const Grid &g, const Surface &s, const BetaModel &beta,
-                        const BuildOptions &opt, SetupResult &out) {
-  out = SetupResult();
-  out.grid = g;
-  out.mib = opt.mib;
-  out.accuracy = opt.accuracy;
-  if (opt.mib != 1 && opt.mib != 2) return "MIB method must be 1 (L1) or 2 (L2)";
-  if (opt.accuracy != 2) return "only 2nd-order spatial accuracy is supported";
-  for (int a = 0; a < 3; a++)
-    if (g.n[a] < 5) return "each axis needs at least 5 nodes";
-  // classification (mesh.cc:51-69)
-  out.ind.resize(g.nodes());
+std::string classify_nodes(const Grid &g, const Surface &s, std::vector<int8_t> &ind) {
+  // mesh.cc:51-69: set_node > 1e-12 outside, < 1e-12 inside, equal: exit
+  ind.resize(g.nodes());
   for (int iz = 0; iz < g.n[2]; iz++)
     for (int iy = 0; iy < g.n[1]; iy++)
       for (int ix = 0; ix < g.n[0]; ix++) {
@@ -699,8 +690,25 @@ std::string build_setup(const Grid &g, const Surface &s, const BetaModel &beta,
           std::snprintf(buf, sizeof(buf), "interface on grid node (%d, %d, %d)", ix, iy, iz);
           return buf;
         }
-        out.ind[g.idx(ix, iy, iz)] = t;
+        ind[g.idx(ix, iy, iz)] = t;
       }
+  return std::string();
+}
+
+std::string build_setup(const Grid &g, const Surface &s, const BetaModel &beta,
+                        const BuildOptions &opt, SetupResult &out) {
+  out = SetupResult();
+  out.grid = g;
+  out.mib = opt.mib;
+  out.accuracy = opt.accuracy;
+  if (opt.mib != 1 && opt.mib != 2) return "MIB method must be 1 (L1) or 2 (L2)";
+  if (opt.accuracy != 2) return "only 2nd-order spatial accuracy is supported";
+  for (int a = 0; a < 3; a++)
+    if (g.n[a] < 5) return "each axis needs at least 5 nodes";
+  {
+    std::string err = classify_nodes(g, s, out.ind);
+    if (!err.empty()) return err;
+  }
   Builder b{g, s, beta, out, std::string()};
   for (int d = 0; d < 3; d++) {
     int ja, ka;

@@ -186,6 +186,10 @@ struct BuildOptions {
   int accuracy = 2;  // only 2nd order is on the hot path
 };
 
+// Node classification of Mesh::Mesh (mesh/mesh.cc:51-69): +1 outside, -1
+// inside, x-fastest.  Returns "" or the node the interface passes through.
+std::string classify_nodes(const Grid &g, const Surface &s, std::vector<int8_t> &ind);
+
 // Returns "" on success, else the reason (the reference would have exited).
 std::string build_setup(const Grid &g, const Surface &s, const BetaModel &beta,
                         const BuildOptions &opt, SetupResult &out);

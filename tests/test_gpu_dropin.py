@@ -1,0 +1,65 @@
+"""The C++ drop-in classes and the config.txt driver on the GPU.
+
+The driver runs the 33^3 Douglas-ADI MIB-L1 case whose reference output is
+held in tests/golden/cfg0_adi_mib1.npz (steps, final time, L-inf, L2,
+argmax): its report must carry the same numbers at the precision it prints.
+"""
+import json
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import PKG_DIR, load_golden
+
+pytestmark = pytest.mark.gpu
+BUILD = os.path.join(PKG_DIR, "_build")
+
+
+def test_cpp_solver_classes():
+    print("dropin_check gpu", flush=True)
+    p = subprocess.run([os.path.join(BUILD, "dropin_check"), "gpu"], capture_output=True,
+                       text=True, timeout=240)
+    assert p.returncode == 0, p.stdout + p.stderr
+    checks = {c["check"]: c for c in map(json.loads, p.stdout.splitlines())}
+    assert set(checks) == {"lod_host_sync_same_field", "lod_same_as_c_abi", "ts_same_as_c_abi",
+                           "adi_corners_refused"}
+    bad = [c for c in checks.values() if not c["ok"]]
+    assert not bad, bad
+
+
+def test_driver_reproduces_the_reference_report(tmp_path):
+    g = load_golden("cfg0_adi_mib1")
+    meta = g["meta"]
+    s = meta["surface"]
+    cfg = tmp_path / "config.txt"
+    cfg.write_text("\n".join([
+        "x-max = 1", "x-min = -1", "y-max = 1", "y-min = -1", "z-max = 1", "z-min = -1",
+        f"x-grid = {meta['n'] + 1}", f"y-grid = {meta['n'] + 1}", f"z-grid = {meta['n'] + 1}",
+        "time-start = 0", f"time-finish = {meta['T']!r}", f"time-step = {meta['dt']!r}",
+        f"surface = ellipsoid({s[1]!r},{s[2]!r},{s[3]!r},{s[4]!r})",
+        "temporal-method = adi", "equation = exp-sin-cos",
+        f"beta = const({meta['beta'][1]},{meta['beta'][2]})",
+        "spatial-method = mib-v1", "spatial-accuracy = 2", ""]))
+    out = tmp_path / "out.txt"
+    print("mts_driver cfg0", flush=True)
+    p = subprocess.run([os.path.join(BUILD, "mts_driver"), "-i", str(cfg), "-o", str(out)],
+                       capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout + p.stderr
+    text = out.read_text()
+    assert "Error of Douglas-ADI with MIB" in text
+    steps, t_end, linf, l2, ax, ay, az = g["summary"][:7]
+    m = re.search(r"T = (\S+)", text)
+    assert float(m.group(1)) == float(f"{t_end:.1e}")
+    # The sphere problem is symmetric under (x, z) -> (-x, -z): the errors at a
+    # node and its mirror image agree to rounding, so which one is the first
+    # maximum follows the last bit of each.
+    m = re.search(r"\[Xmax,Ymax,Zmax\] = \[(\d+),(\d+),(\d+)\]", text)
+    n1 = meta["n"]
+    assert tuple(map(int, m.groups())) in {(int(ax), int(ay), int(az)),
+                                           (n1 - int(ax), int(ay), n1 - int(az))}
+    m = re.search(r"Lmax = (\S+);  L2 = (\S+)", text)
+    assert m.group(1) == f"{linf:.3e}" and m.group(2) == f"{l2:.3e}"
+    assert "CPU time cost" in text
