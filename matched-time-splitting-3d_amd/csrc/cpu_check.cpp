@@ -76,13 +76,15 @@ void emulate_line(Emu &E, const LineArgs &p) {
           s0 = p.S[t.e0.gi];
           s1 = p.S[t.e0.gi + 1];
         }
-        tA[t.e0.sl] = tile_in_value<DIR>(p, t.e0, k, a0);
+        tA[t.e0.sl] = a0;
         if (HAS_S) tB[t.e0.sl] = s0;
         if (t.e1.ok) {
-          tA[t.e1.sl] = tile_in_value<DIR>(p, t.e1, k, a1);
+          tA[t.e1.sl] = a1;
           if (HAS_S) tB[t.e1.sl] = s1;
         }
       }
+      if (p.in_analytic)
+        for (int e = 0; e < 2 * T; e++) tile_end_value<DIR>(p, j0, k, e, tA.get());
       for (int wave = 0; wave < 4; wave++)
           for (int lane = 0; lane < 64; lane++) {
             const int kseg = lane & (L - 1);

@@ -438,17 +438,19 @@ MTS_HD TileAt tile_at(const LineArgs &p, int j0, int k, int e, unsigned magic_n)
   t.sl = l * p.q.lds_len + lds_slot(p.q, t.r);
   return t;
 }
-// The value staged for an input element: the analytic boundary value on the
-// line's end rows when the stage asks for it, else what was read.
+// End row e of the tile's lines (line e / 2; row 0 for even e, n - 1 for
+// odd): the analytic boundary value at the stage's input time, for stages
+// whose input end rows are boundary nodes the previous stage did not write.
 template <int DIR>
-MTS_HD double tile_in_value(const LineArgs &p, const TileAt &t, int k, double v) {
+MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, double *tA) {
   const int n = p.g.n[DIR];
-  if (p.in_analytic && (t.r == 0 || t.r == n - 1)) {
-    int ix, iy, iz;
-    LineMap<DIR>::node(t.j, k, t.r, ix, iy, iz);
-    v = outer_u(p.g, p.e_in, ix, iy, iz);
-  }
-  return v;
+  const int l = e >> 1;
+  const int j = j0 + l;
+  if (j >= LineMap<DIR>::nj(p.g)) return;
+  const int r = (e & 1) ? n - 1 : 0;
+  int ix, iy, iz;
+  LineMap<DIR>::node(j, k, r, ix, iy, iz);
+  tA[l * p.q.lds_len + lds_slot(p.q, r)] = outer_u(p.g, p.e_in, ix, iy, iz);
 }
 // Elements per thread that cover a tile: T * n <= 17 * 256 for every split
 // make_line_part returns.
@@ -499,25 +501,6 @@ MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned 
   t.e1.gi = t.e0.gi + 1;
   t.e1.sl = l1 * p.q.lds_len + lds_slot(p.q, r1);
   return t;
-}
-
-// Stage one element of the tile's input field(s) in LDS.
-template <int DIR, bool HAS_S, int T>
-MTS_HD void tile_load(const LineArgs &p, int j0, int k, int e, unsigned magic_n, double *tA,
-                      double *tB) {
-  const TileAt t = tile_at<DIR, T>(p, j0, k, e, magic_n);
-  if (!t.ok) return;
-  tA[t.sl] = tile_in_value<DIR>(p, t, k, p.A[t.gi]);
-  if (HAS_S) tB[t.sl] = p.S[t.gi];
-}
-
-// Write one element of the tile's result back (interior lines only).
-template <int DIR, int T>
-MTS_HD void tile_store(const LineArgs &p, int j0, int k, int e, unsigned magic_n,
-                       const double *tO) {
-  const TileAt t = tile_at<DIR, T>(p, j0, k, e, magic_n);
-  if (!t.ok || t.j < 1 || t.j > LineMap<DIR>::nj(p.g) - 2) return;
-  p.out[t.gi] = tO[t.sl];
 }
 
 // Per-line table entries a lane needs, read once up front.

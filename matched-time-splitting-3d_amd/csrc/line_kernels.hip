@@ -98,8 +98,8 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
 #pragma unroll
     for (int i = 2; i < MMAX; i++) {
       if (i < m) {
-        const double r = rcp_nr(b[i] - a[i] * c[i - 1]);
-        d[i] = (d[i] - a[i] * d[i - 1]) * r;
+        const double r = rcp_nr(fma(-a[i], c[i - 1], b[i]));
+        d[i] = fma(-a[i], d[i - 1], d[i]) * r;
         a[i] = -a[i] * a[i - 1] * r;
         c[i] = c[i] * r;
         A2 = a[i];
@@ -110,8 +110,8 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
 #pragma unroll
     for (int i = MMAX - 3; i >= 1; i--) {
       if (i <= m - 3) {
-        d[i] = d[i] - c[i] * d[i + 1];
-        a[i] = a[i] - c[i] * a[i + 1];
+        d[i] = fma(-c[i], d[i + 1], d[i]);
+        a[i] = fma(-c[i], a[i + 1], a[i]);
         c[i] = -c[i] * c[i + 1];
       }
     }
@@ -122,9 +122,9 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
       D1 = d[0];
     } else {
       A1 = a[0];
-      B1 = b[0] - c[0] * a[1];
+      B1 = fma(-c[0], a[1], b[0]);
       C1 = -c[0] * c[1];
-      D1 = d[0] - c[0] * d[1];
+      D1 = fma(-c[0], d[1], d[0]);
     }
   }
   const double A1n = grp_down<L>(A1, 1, kseg, 0.0);
@@ -149,8 +149,8 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
     const double k1 = ea * rcp_nr(bm), k2 = ec * rcp_nr(bp);
     const double na = -am * k1;
     const double nc = -cp * k2;
-    const double nb = eb - cm * k1 - ap * k2;
-    const double nd = ed - dm * k1 - dp * k2;
+    const double nb = fma(-ap, k2, fma(-cm, k1, eb));
+    const double nd = fma(-dp, k2, fma(-dm, k1, ed));
     ea = na;
     eb = nb;
     ec = nc;
@@ -161,11 +161,11 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
   // (written as selects of computed values: conditional stores of one value
   // to several elements get merged into one indexed store, which keeps the
   // array in scratch memory)
-  const double xs = (D1 - A1 * xprev - C1 * xe) * i1;
+  const double xs = fma(-C1, xe, fma(-A1, xprev, D1)) * i1;
   const bool act = m >= 2;
 #pragma unroll
   for (int i = 0; i < MMAX; i++) {
-    const double mid = d[i] - a[i] * xs - c[i] * xe;
+    const double mid = fma(-c[i], xe, fma(-a[i], xs, d[i]));
     double v = i == 0 ? xs : (i == m - 1 ? xe : mid);
     d[i] = act && i < m ? v : d[i];
   }
@@ -276,14 +276,19 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     for (int c = 0; c < kChunk; c++) {
       const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
       if (t.e0.ok) {
-        tA[t.e0.sl] = tile_in_value<DIR>(p, t.e0, k, va[c].x);
+        tA[t.e0.sl] = va[c].x;
         if (HAS_S) tB[t.e0.sl] = vb[c].x;
       }
       if (t.e1.ok) {
-        tA[t.e1.sl] = tile_in_value<DIR>(p, t.e1, k, va[c].y);
+        tA[t.e1.sl] = va[c].y;
         if (HAS_S) tB[t.e1.sl] = vb[c].y;
       }
     }
+  }
+  if (p.in_analytic) {
+    // the lines' end rows take the analytic boundary value (2 per line)
+    __syncthreads();
+    if (threadIdx.x < 2 * T) tile_end_value<DIR>(p, j0, k, threadIdx.x, tA);
   }
   __syncthreads();
 

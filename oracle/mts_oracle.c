@@ -226,8 +226,9 @@ static int gtsv(int n, const double *a, const double *b, const double *c, const 
   return 0;
 }
 
-/* Model of the GPU's line solve, operation for operation (the GPU forms
- * reciprocals with a refined hardware estimate where this model divides).
+/* Model of the GPU's line solve, operation for operation, with fused
+ * multiply-adds where the GPU fuses (the GPU forms reciprocals with a refined
+ * hardware estimate where this model divides).
  * A line with a row that is not diagonally dominant (|b| < |a| + |c|) is
  * solved with the Thomas algorithm.  Any other line: the boundary rows 0 and
  * n-1 (identity rows) are moved to the right-hand side; the interior rows
@@ -313,8 +314,8 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
     C2[k] = rc[1];
     D2[k] = rd[1];
     for (int i = 2; i < m; i++) {
-      r = 1.0 / (rb[i] - ra[i] * rc[i - 1]);
-      rd[i] = (rd[i] - ra[i] * rd[i - 1]) * r;
+      r = 1.0 / fma(-ra[i], rc[i - 1], rb[i]);
+      rd[i] = fma(-ra[i], rd[i - 1], rd[i]) * r;
       ra[i] = -ra[i] * ra[i - 1] * r;
       rc[i] = rc[i] * r;
       A2[k] = ra[i];
@@ -322,17 +323,17 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
       D2[k] = rd[i];
     }
     for (int i = m - 3; i >= 1; i--) {
-      rd[i] = rd[i] - rc[i] * rd[i + 1];
-      ra[i] = ra[i] - rc[i] * ra[i + 1];
+      rd[i] = fma(-rc[i], rd[i + 1], rd[i]);
+      ra[i] = fma(-rc[i], ra[i + 1], ra[i]);
       rc[i] = -rc[i] * rc[i + 1];
     }
     if (m == 2) {
       A1[k] = ra[0]; B1[k] = rb[0]; C1[k] = rc[0]; D1[k] = rd[0];
     } else {
       A1[k] = ra[0];
-      B1[k] = rb[0] - rc[0] * ra[1];
+      B1[k] = fma(-rc[0], ra[1], rb[0]);
       C1[k] = -rc[0] * rc[1];
-      D1[k] = rd[0] - rc[0] * rd[1];
+      D1[k] = fma(-rc[0], rd[1], rd[0]);
     }
   }
   for (int k = 0; k < L; k++) {
@@ -357,8 +358,8 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
       const double k1 = ea[k] * (1.0 / bm), k2 = ec[k] * (1.0 / bp);
       na[k] = -am * k1;
       nc[k] = -cp * k2;
-      nb[k] = eb[k] - cm * k1 - ap * k2;
-      nd[k] = ed[k] - dm * k1 - dp * k2;
+      nb[k] = fma(-ap, k2, fma(-cm, k1, eb[k]));
+      nd[k] = fma(-dp, k2, fma(-dm, k1, ed[k]));
     }
     memcpy(ea, na, sizeof(double) * L);
     memcpy(eb, nb, sizeof(double) * L);
@@ -371,9 +372,9 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
   for (int k = 0; k < q.p; k++) {
     const int s = 1 + k * q.m, m = k == q.p - 1 ? q.mlast : q.m;
     const double xp = k == 0 ? 0.0 : xe[k - 1];
-    const double xs = (D1[k] - A1[k] * xp - C1[k] * xe[k]) * (1.0 / B1[k]);
+    const double xs = fma(-C1[k], xe[k], fma(-A1[k], xp, D1[k])) * (1.0 / B1[k]);
     U[s] = xs;
-    for (int i = 1; i < m - 1; i++) U[s + i] = d[s + i] - a[s + i] * xs - c[s + i] * xe[k];
+    for (int i = 1; i < m - 1; i++) U[s + i] = fma(-c[s + i], xe[k], fma(-a[s + i], xs, d[s + i]));
     U[s + m - 1] = xe[k];
   }
   free(w);

@@ -101,8 +101,11 @@ def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
         print(f"  step {s + 1}: vs oracle (Thomas) {e_ref:.2e}, vs CPU model of the "
               f"GPU line solve {e_model:.2e}", flush=True)
         assert e_ref <= growth_tol(uo), f"step {s + 1}"
-        # the same elimination order as the kernel: rounding of the right-hand side only
-        assert e_model <= 0.1 * growth_tol(uo), f"step {s + 1} (model)"
+        # the same elimination order as the kernel, so what is left is the
+        # rounding of the right-hand side (the kernel applies each MIB block as
+        # one precombined linear form), amplified by TS's explicit half-steps
+        # (dt beta / h^2 = 320 on the first case)
+        assert e_model <= 0.25 * growth_tol(uo), f"step {s + 1} (model)"
         u = ug
 
 
