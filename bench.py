@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--n", type=int, default=256, help="intervals per axis")
     ap.add_argument("--method", default="lod-cn", choices=["adi", "lod-ie", "lod-cn", "ts"])
     ap.add_argument("--mib", type=int, default=2, choices=[1, 2])
+    ap.add_argument("--fp32-form", type=int, default=1,
+                    help="also time the FP32 form on the same workload (0 to skip)")
     ap.add_argument("--cpu-baseline", type=int, default=1,
                     help="time the checker restatement on the host (0 to skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -127,6 +129,10 @@ def main():
         if key in tr:
             traffic = tr[key]
 
+    fp32 = None
+    if args.fp32_form and args.method != "adi":
+        fp32 = fp32_form(mts, ctx, st, args, dt, nodes, value, linf)
+
     cpu = None
     if args.cpu_baseline:
         cpu = cpu_baseline(mts, st, args.method, dt, nodes, phys)
@@ -168,9 +174,49 @@ def main():
             "avg_launch_us": avg_s * 1e6,
         },
         "cpu_baseline": cpu,
+        "fp32_form": fp32,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
+    """The same K steps in the FP32 form (FP32 fields and line solve, rows
+    formed in FP64), with its L-inf error against the exact solution."""
+    inst = mts.Instance(ctx, st, args.method, dt, precision="fp32")
+    inst.initialize(0.0)
+    t = 0.0
+    for _ in range(args.warmup):
+        t += dt
+        inst.step(t)
+    inst.sync()
+    inst.initialize(0.0)
+    inst.profile(True)
+    inst.profile_reset()
+    inst.sync()
+    t = 0.0
+    t_start = time.perf_counter()
+    for _ in range(args.steps):
+        t += dt
+        inst.step(t)
+    inst.sync()
+    elapsed = time.perf_counter() - t_start
+    stages = inst.profile_read()
+    inst.profile(False)
+    linf, l2, _ = inst.error(t)
+    inst.close()
+    value = nodes * args.steps / elapsed
+    return {
+        "value": value,
+        "unit": "grid-pt-steps/s",
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "dtype": "f32 fields and solve, f64 rows",
+        "linf_error": linf,
+        "l2_error": l2,
+        "linf_vs_fp64": linf / fp64_linf,
+        "speedup_vs_fp64": value / fp64_value,
+        "stage_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in stages.items()},
+    }
 
 
 METRIC = "grid-pt-steps/s at fixed L-inf error (FP32+batch vs FP64); solves/s + p50/p99"

@@ -32,6 +32,10 @@ enum {
 
 /* temporal methods (constant.h: Temporal_Method_Type) */
 enum { MTS_ADI = 0, MTS_LOD_IE = 1, MTS_LOD_CN = 2, MTS_TS = 3 };
+/* field precision: FP64 (the reference's arithmetic) or the FP32 form (FP32
+ * field storage and line solve, rows formed in FP64, FP64 setup and norms;
+ * LOD-IE, LOD-CN and TS) */
+enum { MTS_FP64 = 0, MTS_FP32 = 1 };
 /* surfaces (surface/ *) */
 enum { MTS_SURF_ELLIPSOID = 0, MTS_SURF_TORUS = 1, MTS_SURF_BUMPY = 2, MTS_SURF_CALLBACK = 3 };
 /* diffusion coefficients: beta- = p0, beta+ = p1 (CONST); beta- = p0 (1 + sin x cos y / 2) (SINCOS) */
@@ -147,6 +151,13 @@ int mts_ctx_device_info(mts_ctx *ctx, char *name, size_t name_len, size_t *hbm_b
 int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
                         int method, double dt, mts_instance **out);
 void mts_instance_destroy(mts_instance *inst);
+/* mts_instance_create with a field precision (MTS_FP64 or MTS_FP32).  No
+ * counterpart in the reference (FP64 throughout); the FP32 form halves the
+ * field's HBM traffic and registers.  Douglas-ADI is FP64 only
+ * (MTS_E_DISABLED). */
+int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                           int method, double dt, int precision, mts_instance **out);
+int mts_instance_precision(const mts_instance *inst, int *precision);
 
 /* Field transfer (host <-> the instance's solution field). */
 int mts_field_upload(mts_instance *inst, const double *host, int layout);
@@ -174,8 +185,9 @@ int mts_sync(mts_instance *inst);
  * (Error, douglas_adi.cc:1723-1770). */
 int mts_error_norms(mts_instance *inst, double t, double *linf, double *l2, int argmax[3]);
 
-/* Device pointer of the solution field (x pitch in *pitch, in doubles) and the
- * instance's HIP stream, for callers that keep data on the device. */
+/* Device pointer of the solution field (x pitch in *pitch, in elements) and
+ * the instance's HIP stream, for callers that keep data on the device.  For an
+ * MTS_FP32 instance the pointer addresses floats. */
 int mts_field_device(mts_instance *inst, double **dptr, int *pitch, void **stream);
 
 /* Per-stage timing with HIP events on the instance's stream (the reference
@@ -221,6 +233,7 @@ typedef struct {
   int method;                 /* MTS_ADI .. MTS_TS */
   double dt, t0, t_finish;
   long long tag;              /* returned in the result */
+  int precision;              /* MTS_FP64 or MTS_FP32 */
 } mts_request;
 
 typedef struct {

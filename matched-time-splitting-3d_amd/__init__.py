@@ -32,6 +32,8 @@ METHODS = {"adi": ADI, "lod-ie": LOD_IE, "lod-cn": LOD_CN, "ts": TS}
 SURF_ELLIPSOID, SURF_TORUS, SURF_BUMPY, SURF_CALLBACK = 0, 1, 2, 3
 BETA_CONST, BETA_SINCOS = 0, 1
 LAYOUT_XFAST, LAYOUT_REF = 0, 1
+FP64, FP32 = 0, 1
+PRECISIONS = {"fp64": FP64, "fp32": FP32}
 
 
 class MtsError(RuntimeError):
@@ -80,7 +82,7 @@ class SetupInfo(C.Structure):
 class Request(C.Structure):
     _fields_ = [("grid", GridDesc), ("surface", SurfaceDesc), ("physics", PhysicsDesc),
                 ("mib", C.c_int), ("method", C.c_int), ("dt", C.c_double), ("t0", C.c_double),
-                ("t_finish", C.c_double), ("tag", C.c_longlong)]
+                ("t_finish", C.c_double), ("tag", C.c_longlong), ("precision", C.c_int)]
 
 
 class Result(C.Structure):
@@ -137,6 +139,8 @@ def lib() -> C.CDLL:
             "mts_profile_reset": ([VP], I),
             "mts_profile_read": ([VP, VP, I, P(I)], I),
             "mts_instance_dt": ([VP, P(D)], I),
+            "mts_instance_create_ex": ([VP, VP, P(PhysicsDesc), I, D, I, P(VP)], I),
+            "mts_instance_precision": ([VP, P(I)], I),
             "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
             "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
             "mts_server_create": ([VP, C.c_size_t, I, I, P(VP)], I),
@@ -296,13 +300,16 @@ class Context:
 class Instance:
     """Douglas_ADI / LOD / TS solver resident on the GPU."""
 
-    def __init__(self, ctx: Context, setup: Setup, method, dt: float):
+    def __init__(self, ctx: Context, setup: Setup, method, dt: float, precision="fp64"):
         self._h = C.c_void_p()
         self.ctx, self.setup = ctx, setup
         self.method = METHODS[method] if isinstance(method, str) else int(method)
         self.dt = float(dt)
-        _check(lib().mts_instance_create(ctx.handle, setup.handle, C.byref(setup.physics),
-                                         self.method, self.dt, C.byref(self._h)))
+        self.precision = (PRECISIONS[precision] if isinstance(precision, str)
+                          else int(precision))
+        _check(lib().mts_instance_create_ex(ctx.handle, setup.handle, C.byref(setup.physics),
+                                            self.method, self.dt, self.precision,
+                                            C.byref(self._h)))
         self.shape = setup.shape
 
     def close(self):
@@ -392,10 +399,12 @@ def batch_run(instances, t0, t_finish):
 
 
 def request(g: GridDesc, surf: SurfaceDesc, phys: PhysicsDesc, method, dt: float,
-            t_finish: float, mib: int = 1, t0: float = 0.0, tag: int = 0) -> Request:
+            t_finish: float, mib: int = 1, t0: float = 0.0, tag: int = 0,
+            precision="fp64") -> Request:
     """An ``mts_request`` for :meth:`Server.submit`."""
     m = METHODS[method] if isinstance(method, str) else int(method)
-    return Request(g, surf, phys, mib, m, dt, t0, t_finish, tag)
+    pr = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+    return Request(g, surf, phys, mib, m, dt, t0, t_finish, tag, pr)
 
 
 class Server:

@@ -143,7 +143,11 @@ struct mts_instance {
   size_t field_elems = 0;
   StreamHolder sh;                // the instance's stream (outlives the buffers below)
   hipStream_t stream = nullptr;   // == sh.s
-  DevBuf<double> u, r;            // solution; Douglas stage scratch
+  int precision = MTS_FP64;       // field storage and line solve
+  DevBuf<double> u, r;            // solution (FP64 form); Douglas stage scratch
+  DevBuf<float> uf;               // solution (FP32 form)
+  bool f32() const { return precision == MTS_FP32; }
+  void *field() { return f32() ? static_cast<void *>(uf.p) : static_cast<void *>(u.p); }
   DevBuf<int8_t> ind;
   DevBuf<double> tab[3];
   DevBuf<int> blk_ptr[3];
@@ -180,7 +184,7 @@ struct mts_instance {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
   size_t device_bytes() const {
-    size_t b = u.bytes() + r.bytes() + ind.bytes() + spec.bytes() + part_max.bytes() +
+    size_t b = u.bytes() + uf.bytes() + r.bytes() + ind.bytes() + spec.bytes() + part_max.bytes() +
                part_sq.bytes() + part_idx.bytes();
     for (int d = 0; d < 3; d++)
       b += tab[d].bytes() + blk_ptr[d].bytes() + blk_hdr[d].bytes() + blk_val[d].bytes() +
@@ -252,6 +256,7 @@ hipError_t timed(mts_instance *in, const char *name, F launch) {
 // footprint before admitting the solve.
 struct mts::InstancePlan {
   int method = 0;
+  int precision = MTS_FP64;
   int min_lanes = 16;
   double dt = 0.0, theta_dt = 0.0;
   mts::Grid grid;
@@ -268,9 +273,12 @@ void mts::PlanDeleter::operator()(mts::InstancePlan *p) const { delete p; }
 size_t mts::plan_device_bytes(const mts::InstancePlan &p) { return p.device_bytes; }
 
 int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, int method,
-                       double dt, mts::PlanPtr &out) {
+                       double dt, int precision, mts::PlanPtr &out) {
   if (!setup || !physics) return fail(MTS_E_ARG, "null argument");
   if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
+  if (precision != MTS_FP64 && precision != MTS_FP32) return fail(MTS_E_ARG, "unknown precision");
+  if (precision == MTS_FP32 && method == MTS_ADI)
+    return fail(MTS_E_DISABLED, "the FP32 form covers LOD-IE, LOD-CN and TS; Douglas-ADI runs in FP64");
   if (!(dt > 0.0)) return fail(MTS_E_ARG, "dt must be positive");
   mts::BetaModel beta;
   std::string err;
@@ -290,6 +298,7 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
   try {
     mts::PlanPtr p(new mts::InstancePlan());
     p->method = method;
+    p->precision = precision;
     p->dt = dt;
     p->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
     p->grid = setup->res.grid;
@@ -337,7 +346,8 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       p->recs.push_back({mts_instance::rec_key(kd.first, kd.second), std::move(rec)});
     }
     // exactly what instance_from_plan allocates
-    size_t b = p->field_elems * (sizeof(double) * (method == MTS_ADI ? 2 : 1) + sizeof(int8_t));
+    size_t b = p->field_elems * ((precision == MTS_FP32 ? sizeof(float) : sizeof(double)) +
+                                 (method == MTS_ADI ? sizeof(double) : 0) + sizeof(int8_t));
     for (int d = 0; d < 3; d++)
       b += t.tab[d].size() * sizeof(double) + t.blk_ptr[d].size() * sizeof(int) +
            t.blk_hdr[d].size() * sizeof(mts::BlockHdr) + t.blk_val[d].size() * sizeof(double) +
@@ -372,6 +382,7 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   std::unique_ptr<mts_instance> in(new mts_instance());
   in->ctx = ctx;
   in->method = p.method;
+  in->precision = p.precision;
   in->dt = p.dt;
   in->theta_dt = p.theta_dt;
   in->grid = g;
@@ -389,8 +400,13 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   // returns) is not ordered before the kernels launched on it.  The plan's
   // host staging stays alive until the stream is synchronized at the end.
   const hipStream_t st = in->stream;
-  MTS_HIP(in->u.alloc(in->field_elems, st));
-  MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
+  if (in->f32()) {
+    MTS_HIP(in->uf.alloc(in->field_elems, st));
+    MTS_HIP(hipMemsetAsync(in->uf.p, 0, in->field_elems * sizeof(float), st));
+  } else {
+    MTS_HIP(in->u.alloc(in->field_elems, st));
+    MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
+  }
   if (p.method == MTS_ADI) {
     MTS_HIP(in->r.alloc(in->field_elems, st));
     MTS_HIP(hipMemsetAsync(in->r.p, 0, in->field_elems * sizeof(double), st));
@@ -641,9 +657,25 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
   if (!ctx || !out) return fail(MTS_E_ARG, "null argument");
   *out = nullptr;
   mts::PlanPtr plan;
-  int rc = mts::plan_instance(setup, physics, method, dt, plan);
+  int rc = mts::plan_instance(setup, physics, method, dt, MTS_FP64, plan);
   if (rc) return rc;
   return mts::instance_from_plan(ctx, std::move(plan), out);
+}
+
+int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                           int method, double dt, int precision, mts_instance **out) {
+  if (!ctx || !out) return fail(MTS_E_ARG, "null argument");
+  *out = nullptr;
+  mts::PlanPtr plan;
+  int rc = mts::plan_instance(setup, physics, method, dt, precision, plan);
+  if (rc) return rc;
+  return mts::instance_from_plan(ctx, std::move(plan), out);
+}
+
+int mts_instance_precision(const mts_instance *in, int *precision) {
+  if (!in || !precision) return fail(MTS_E_ARG, "null argument");
+  *precision = in->precision;
+  return MTS_OK;
 }
 
 void mts_instance_destroy(mts_instance *inst) {
@@ -675,6 +707,13 @@ int mts_field_upload(mts_instance *in, const double *host, int layout) {
   MTS_HIP(hipSetDevice(in->ctx->device));
   std::vector<double> buf;
   pack_field(in, host, layout, buf);
+  if (in->f32()) {
+    std::vector<float> bf(buf.begin(), buf.end());
+    MTS_HIP(hipMemcpyAsync(in->uf.p, bf.data(), bf.size() * sizeof(float), hipMemcpyHostToDevice,
+                           in->stream));
+    MTS_HIP(hipStreamSynchronize(in->stream));
+    return MTS_OK;
+  }
   MTS_HIP(hipMemcpyAsync(in->u.p, buf.data(), buf.size() * sizeof(double), hipMemcpyHostToDevice,
                          in->stream));
   MTS_HIP(hipStreamSynchronize(in->stream));
@@ -686,9 +725,17 @@ int mts_field_download(mts_instance *in, double *host, int layout) {
   if (layout != MTS_LAYOUT_XFAST && layout != MTS_LAYOUT_REF) return fail(MTS_E_ARG, "bad layout");
   MTS_HIP(hipSetDevice(in->ctx->device));
   std::vector<double> buf(in->field_elems);
-  MTS_HIP(hipMemcpyAsync(buf.data(), in->u.p, buf.size() * sizeof(double), hipMemcpyDeviceToHost,
-                         in->stream));
-  MTS_HIP(hipStreamSynchronize(in->stream));
+  if (in->f32()) {
+    std::vector<float> bf(in->field_elems);
+    MTS_HIP(hipMemcpyAsync(bf.data(), in->uf.p, bf.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           in->stream));
+    MTS_HIP(hipStreamSynchronize(in->stream));
+    std::copy(bf.begin(), bf.end(), buf.begin());
+  } else {
+    MTS_HIP(hipMemcpyAsync(buf.data(), in->u.p, buf.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, in->stream));
+    MTS_HIP(hipStreamSynchronize(in->stream));
+  }
   const mts::Grid &g = in->grid;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   for (int iz = 0; iz < nz; iz++)
@@ -706,7 +753,7 @@ int mts_field_download(mts_instance *in, double *host, int layout) {
 int mts_initialize(mts_instance *in, double t0) {
   if (!in) return fail(MTS_E_ARG, "null instance");
   MTS_HIP(hipSetDevice(in->ctx->device));
-  MTS_HIP(mts::launch_init(in->geom, in->u.p, std::exp(-t0), in->stream));
+  MTS_HIP(mts::launch_init(in->geom, in->field(), in->f32(), std::exp(-t0), in->stream));
   return MTS_OK;
 }
 
@@ -727,7 +774,7 @@ static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::L
   mts::LineArgs a = args;
   a.mib.rec = in->rec_for(dir, flags);
   if (!a.mib.rec && in->mib[dir].hdr) return hipErrorInvalidValue;
-  return mts::launch_line(dir, flags, a, st);
+  return mts::launch_line(dir, flags, in->f32(), a, st);
 }
 
 int mts_step(mts_instance *in, double t_next) {
@@ -771,15 +818,15 @@ int mts_step(mts_instance *in, double t_next) {
     a.out = in->u.p;
     MTS_HIP(timed(in, "adi_z", [&] { return line_launch(in, 2, kRhsDouglas | kFlagSolve, a, st); }));
     MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
+      return launch_faces(in->geom, in->phys, in->u.p, false, e1, 0.0, st);
     }));
   } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
     const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
-    a.A = in->u.p;
-    a.out = in->u.p;
+    a.A = in->field();
+    a.out = in->field();
     a.mib = in->mib[0];
     MTS_HIP(timed(in, "lod_x", [&] { return line_launch(in, 0, rhs | kFlagSolve | kFlagLhsJump, a, st); }));
     a.in_analytic = 1;
@@ -793,15 +840,15 @@ int mts_step(mts_instance *in, double t_next) {
       return line_launch(in, 2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
     }));
     MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->u.p, e1, dt, st);
+      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, dt, st);
     }));
   } else {
     const double en = std::exp(-(t_next - dt));
     const double eh = std::exp(-(t_next - 0.5 * dt));
     LineArgs a = base_args(in);
     a.e_jump = e1;
-    a.A = in->u.p;
-    a.out = in->u.p;
+    a.A = in->field();
+    a.out = in->field();
     // explicit (I + dt/2 L_x), (I + dt/2 L_y) with boundary values at t^{n+1/2}
     a.e_out = eh;
     a.mib = in->mib[0];
@@ -825,7 +872,7 @@ int mts_step(mts_instance *in, double t_next) {
     a.mib = in->mib[0];
     MTS_HIP(timed(in, "ts_ix", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->u.p, e1, 0.0, st);
+      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, 0.0, st);
     }));
   }
   return MTS_OK;
@@ -856,7 +903,7 @@ int mts_error_norms(mts_instance *in, double t, double *linf, double *l2, int ar
   if (!in) return fail(MTS_E_ARG, "null instance");
   MTS_HIP(hipSetDevice(in->ctx->device));
   int nparts = 0;
-  MTS_HIP(mts::launch_norms(in->geom, in->u.p, std::exp(-t), in->part_max.p, in->part_idx.p,
+  MTS_HIP(mts::launch_norms(in->geom, in->field(), in->f32(), std::exp(-t), in->part_max.p, in->part_idx.p,
                             in->part_sq.p, &nparts, in->stream));
   std::vector<double> pm(nparts), ps(nparts);
   std::vector<long long> pi(nparts);
@@ -902,7 +949,7 @@ int mts_instance_device_bytes(const mts_instance *in, size_t *bytes) {
 
 int mts_field_device(mts_instance *in, double **dptr, int *pitch, void **stream) {
   if (!in) return fail(MTS_E_ARG, "null instance");
-  if (dptr) *dptr = in->u.p;
+  if (dptr) *dptr = static_cast<double *>(in->field());  // float storage for an FP32 instance
   if (pitch) *pitch = in->px;
   if (stream) *stream = in->stream;
   return MTS_OK;

@@ -21,7 +21,7 @@ struct PlanDeleter {
 using PlanPtr = std::unique_ptr<InstancePlan, PlanDeleter>;
 
 int plan_instance(const mts_setup *setup, const mts_physics_desc *physics, int method, double dt,
-                  PlanPtr &out);
+                  int precision, PlanPtr &out);
 size_t plan_device_bytes(const InstancePlan &p);
 // The device half: allocates, uploads and returns the instance.
 int instance_from_plan(mts_ctx *ctx, PlanPtr plan, mts_instance **out);

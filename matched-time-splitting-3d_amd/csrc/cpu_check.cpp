@@ -70,11 +70,13 @@ void emulate_line(Emu &E, const LineArgs &p) {
         const TilePair t = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
         if (!t.e0.ok) continue;
         if (t.e0.gi % 2 != 0) std::abort();  // 16-byte alignment
-        const double a0 = p.A[t.e0.gi], a1 = p.A[t.e0.gi + 1];
+        const double *fA = static_cast<const double *>(p.A);
+        const double *fS = static_cast<const double *>(p.S);
+        const double a0 = fA[t.e0.gi], a1 = fA[t.e0.gi + 1];
         double s0 = 0.0, s1 = 0.0;
         if (HAS_S) {
-          s0 = p.S[t.e0.gi];
-          s1 = p.S[t.e0.gi + 1];
+          s0 = fS[t.e0.gi];
+          s1 = fS[t.e0.gi + 1];
         }
         tA[t.e0.sl] = a0;
         if (HAS_S) tB[t.e0.sl] = s0;
@@ -110,8 +112,8 @@ void emulate_line(Emu &E, const LineArgs &p) {
         const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
         const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
         const double v0 = t.e0.ok ? tO[t.e0.sl] : 0.0, v1 = t.e1.ok ? tO[t.e1.sl] : 0.0;
-        if (w0) p.out[t.e0.gi] = v0;
-        if (w1) p.out[t.e1.gi] = v1;
+        if (w0) static_cast<double *>(p.out)[t.e0.gi] = v0;
+        if (w1) static_cast<double *>(p.out)[t.e1.gi] = v1;
       }
     }
 }

@@ -441,8 +441,8 @@ MTS_HD TileAt tile_at(const LineArgs &p, int j0, int k, int e, unsigned magic_n)
 // End row e of the tile's lines (line e / 2; row 0 for even e, n - 1 for
 // odd): the analytic boundary value at the stage's input time, for stages
 // whose input end rows are boundary nodes the previous stage did not write.
-template <int DIR>
-MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, double *tA) {
+template <int DIR, typename R>
+MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, R *tA) {
   const int n = p.g.n[DIR];
   const int l = e >> 1;
   const int j = j0 + l;
@@ -450,7 +450,7 @@ MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, double *tA) 
   const int r = (e & 1) ? n - 1 : 0;
   int ix, iy, iz;
   LineMap<DIR>::node(j, k, r, ix, iy, iz);
-  tA[l * p.q.lds_len + lds_slot(p.q, r)] = outer_u(p.g, p.e_in, ix, iy, iz);
+  tA[l * p.q.lds_len + lds_slot(p.q, r)] = (R)outer_u(p.g, p.e_in, ix, iy, iz);
 }
 // Elements per thread that cover a tile: T * n <= 17 * 256 for every split
 // make_line_part returns.
@@ -684,8 +684,8 @@ MTS_HD BlockRhs block_rhs(const LineArgs &p, int j, int k, const LineMeta &lm,
 }
 
 // The block's five input rows ll-1 .. ll+3 from LDS (0 past the line's end).
-template <int FLAGS>
-MTS_HD void block_inputs(const LineArgs &p, int ll, const double *la, const double *lb,
+template <int FLAGS, typename R>
+MTS_HD void block_inputs(const LineArgs &p, int ll, const R *la, const R *lb,
                          double (&fa)[5], double (&fs)[5]) {
   constexpr bool HAS_S = (FLAGS & 3) == kRhsDouglas;
 #pragma unroll
@@ -693,8 +693,8 @@ MTS_HD void block_inputs(const LineArgs &p, int ll, const double *la, const doub
     const int r = ll - 1 + t;
     const bool in = r <= p.q.n - 1;
     const int sl = lds_slot(p.q, r);
-    fa[t] = in ? la[sl] : 0.0;
-    fs[t] = HAS_S && in ? lb[sl] : 0.0;
+    fa[t] = in ? (double)la[sl] : 0.0;
+    fs[t] = HAS_S && in ? (double)lb[sl] : 0.0;
   }
 }
 // Right-hand side of one patched row from its stage record.
@@ -722,10 +722,12 @@ MTS_HD double block_row_rhs(const LineArgs &p, const double *w, const double (&f
 // per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
 // couplings of rows 1 and n-2 to the boundary rows stay in a and c.
 // la/lb: the line's input field(s) staged in LDS (slot layout of LinePart).
-template <int DIR, int MMAX, int FLAGS>
-MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const double *la,
-                      const double *lb, const LaneSeg &sg, double (&a)[MMAX],
-                      double (&b)[MMAX], double (&c)[MMAX], double (&d)[MMAX]) {
+// R is the field type (double, or float for the FP32 form): the rows are
+// formed in FP64 and rounded to R once.
+template <int DIR, int MMAX, int FLAGS, typename R = double>
+MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const R *la,
+                      const R *lb, const LaneSeg &sg, R (&a)[MMAX],
+                      R (&b)[MMAX], R (&c)[MMAX], R (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   using M = LineMap<DIR>;
@@ -749,8 +751,6 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
   const LineSource<DIR> src(p.P, g, j, k);
   (void)src;
-  auto FA = [&](int i) { return la[lds_slot(q, i)]; };
-  auto FS = [&](int i) { return lb[lds_slot(q, i)]; };
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
     if (r < m) {
@@ -760,26 +760,26 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
       const bool out = ((cm >> r) & 1u) != 0;
       double vd[3];
       op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
-      a[r] = mtdt * vd[0];
-      b[r] = 1 - tdt * vd[1];
-      c[r] = mtdt * vd[2];
+      a[r] = (R)(mtdt * vd[0]);
+      b[r] = (R)(1 - tdt * vd[1]);
+      c[r] = (R)(mtdt * vd[2]);
       if (RHS == kRhsCopy) {
         d[r] = la[base + r];
       } else {
-        const double *f = RHS == kRhsDouglas ? lb : la;
+        const R *f = RHS == kRhsDouglas ? lb : la;
         const double fp = r == 0 ? f[sl_prev] : f[base + r - 1];
         const double fc = f[base + r];
         const double fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
         const double sum = fp * vd[0] + fc * vd[1] + fn * vd[2];
         if (RHS == kRhsDouglas) {
-          d[r] = -p.dt * sum + la[base + r];
+          d[r] = (R)(-p.dt * sum + (double)la[base + r]);
         } else {
           double v = tdt * sum + fc;
           if (RHS == kRhsTSZ) {
             v = v + p.c_f1 * src.at(p.P, g, p.e_f1, out, j, k, i);
             v = v + p.c_f2 * src.at(p.P, g, p.e_f2, out, j, k, i);
           }
-          d[r] = v;
+          d[r] = (R)v;
         }
       }
     }
@@ -799,18 +799,18 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
     // The right-hand side goes in as a select of register values; a select
     // between a global load and an array element would be turned into a load
     // through a selected pointer, which keeps the array in scratch memory.
-    const double bv0 = bv[0], bv1 = bv[1], bv2 = bv[2];
+    const R bv0 = (R)bv[0], bv1 = (R)bv[1], bv2 = (R)bv[2];
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
       const int rr = s + r - ll;
       const bool hit = r < m && rr >= 0 && rr < nr;
       if (SOLVE && hit) {
         const double *lhs = rec + rr * kRecRow + kRecLhs;
-        a[r] = lhs[0];
-        b[r] = lhs[1];
-        c[r] = lhs[2];
+        a[r] = (R)lhs[0];
+        b[r] = (R)lhs[1];
+        c[r] = (R)lhs[2];
       }
-      const double dv = rr == 0 ? bv0 : (rr == 1 ? bv1 : bv2);
+      const R dv = rr == 0 ? bv0 : (rr == 1 ? bv1 : bv2);
       d[r] = hit ? dv : d[r];
     }
   }
@@ -818,9 +818,9 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
 
 // Epilogue of the lane's rows into the LDS output tile; the group's lane 0
 // also writes the two boundary rows (x0, xn: their values).
-template <int DIR, int MMAX, int FLAGS>
+template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, const LaneSeg &sg,
-                       const double (&x)[MMAX], double x0, double xn, double *lo) {
+                       const R (&x)[MMAX], double x0, double xn, R *lo) {
   constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
   using M = LineMap<DIR>;
   const LinePart &q = p.q;
@@ -834,7 +834,7 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, cons
       if (r < sg.m) {
         double v = x[r];
         if (EPI) v = v + p.c_f1 * src.at(p.P, p.g, p.e_f1, ((cm >> r) & 1u) != 0, j, k, sg.s + r);
-        lo[base + r] = v;
+        lo[base + r] = (R)v;
       }
     }
   }
@@ -847,8 +847,8 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, cons
       M::node(j, k, n - 1, ix, iy, iz);
       xn = xn + p.c_f1 * source(p.P, p.g, p.e_f1, row_outside(p.mib, lm, n - 1), ix, iy, iz);
     }
-    lo[0] = x0;
-    lo[1 + q.p * q.stride] = xn;
+    lo[0] = (R)x0;
+    lo[1 + q.p * q.stride] = (R)xn;
   }
 }
 

@@ -138,9 +138,11 @@ struct LineArgs {
   LinePart q;          // filled at launch from the line length
   int min_lanes;       // lane-group lower bound for the split (16: the default)
   PhysConst P;
-  const double *A;     // stage input
-  const double *S;     // second input (Douglas y/z: u^n), may be null
-  double *out;         // stage output (may alias A or S)
+  // Fields, FP64 (double) or FP32 (float) as the instance stores them; the
+  // kernel instantiation for that type reads and writes them.
+  const void *A;       // stage input
+  const void *S;       // second input (Douglas y/z: u^n), may be null
+  void *out;           // stage output (may alias A or S)
   double theta_dt;     // implicit coefficient
   double dt;
   double e_out;        // exp(-t) of the output boundary rows

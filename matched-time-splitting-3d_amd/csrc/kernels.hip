@@ -168,8 +168,9 @@ __global__ __launch_bounds__(256) void spec_kernel(SpecArgs p) {
 }
 
 // ------------------------------------------------------- boundary faces
-__global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, double *out,
-                                                    double e, double cf) {
+template <typename R>
+__global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *out, double e,
+                                                    double cf) {
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const long long fz = (long long)nx * ny, fy = (long long)nx * nz, fx = (long long)ny * nz;
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -195,22 +196,24 @@ __global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, doub
   const size_t c = node_index(g, ix, iy, iz);
   double v = outer_u(g, e, ix, iy, iz);
   if (cf != 0.0) v = v + cf * source(P, g, e, g.ind[c] > 0, ix, iy, iz);
-  out[c] = v;
+  out[c] = (R)v;
 }
 
 // ------------------------------------------------------- initialization
-__global__ __launch_bounds__(256) void init_kernel(DevGeom g, double *u, double e) {
+template <typename R>
+__global__ __launch_bounds__(256) void init_kernel(DevGeom g, R *u, double e) {
   const int ix = blockIdx.x * blockDim.x + threadIdx.x;
   const int iy = blockIdx.y, iz = blockIdx.z;
   if (ix >= g.n[0]) return;
   const size_t c = node_index(g, ix, iy, iz);
-  u[c] = g.ind[c] > 0 ? outer_u(g, e, ix, iy, iz) : inner_u(g, e, ix, iy, iz);
+  u[c] = (R)(g.ind[c] > 0 ? outer_u(g, e, ix, iy, iz) : inner_u(g, e, ix, iy, iz));
 }
 
 // ------------------------------------------------------------ error norms
 // Reference order index (ix outer, iz inner, douglas_adi.cc:1736-1760) so the
 // first maximum in that order wins ties.
-__global__ __launch_bounds__(256) void norms_kernel(DevGeom g, const double *u, double e,
+template <typename R>
+__global__ __launch_bounds__(256) void norms_kernel(DevGeom g, const R *u, double e,
                                                     double *pmax, long long *pidx,
                                                     double *psq) {
   __shared__ double smax[256];
@@ -228,7 +231,7 @@ __global__ __launch_bounds__(256) void norms_kernel(DevGeom g, const double *u, 
     int iz = (int)(r / ny);
     size_t c = node_index(g, ix, iy, iz);
     double ex = g.ind[c] > 0 ? outer_u(g, e, ix, iy, iz) : inner_u(g, e, ix, iy, iz);
-    double er = fabs(u[c] - ex);
+    double er = fabs((double)u[c] - ex);
     long long ri = ((long long)ix * ny + iy) * nz + iz;
     if (er > mx || (er == mx && ri < mi)) {
       mx = er;
@@ -295,13 +298,13 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
 }  // namespace
 
 // --------------------------------------------------------------- launchers
-hipError_t launch_line(int dir, int flags, const LineArgs &args, hipStream_t s) {
+hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipStream_t s) {
   LineArgs a = args;
   if (!make_line_part(a.g.n[dir], a.q, a.min_lanes)) return hipErrorInvalidValue;
   if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
   const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
 #define MTS_DL(D, LL) \
-  if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, a, s);
+  if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, f32, a, s);
   MTS_DL(0, 16) MTS_DL(0, 32) MTS_DL(0, 64)
   MTS_DL(1, 16) MTS_DL(1, 32) MTS_DL(1, 64)
   MTS_DL(2, 16) MTS_DL(2, 32) MTS_DL(2, 64)
@@ -321,18 +324,26 @@ hipError_t launch_spec(const SpecArgs &a, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_faces(const DevGeom &g, const PhysConst &P, double *out, double e, double cf,
-                        hipStream_t s) {
+hipError_t launch_faces(const DevGeom &g, const PhysConst &P, void *out, bool f32, double e,
+                        double cf, hipStream_t s) {
   long long total = 2LL * ((long long)g.n[0] * g.n[1] + (long long)g.n[0] * g.n[2] +
                            (long long)g.n[1] * g.n[2]);
-  hipLaunchKernelGGL(faces_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, g, P,
-                     out, e, cf);
+  const dim3 grid((unsigned)((total + 255) / 256));
+  if (f32)
+    hipLaunchKernelGGL(faces_kernel<float>, grid, dim3(256), 0, s, g, P, static_cast<float *>(out),
+                       e, cf);
+  else
+    hipLaunchKernelGGL(faces_kernel<double>, grid, dim3(256), 0, s, g, P,
+                       static_cast<double *>(out), e, cf);
   return hipGetLastError();
 }
 
-hipError_t launch_init(const DevGeom &g, double *u, double e, hipStream_t s) {
+hipError_t launch_init(const DevGeom &g, void *u, bool f32, double e, hipStream_t s) {
   dim3 grid((g.n[0] + 255) / 256, g.n[1], g.n[2]);
-  hipLaunchKernelGGL(init_kernel, grid, dim3(256), 0, s, g, u, e);
+  if (f32)
+    hipLaunchKernelGGL(init_kernel<float>, grid, dim3(256), 0, s, g, static_cast<float *>(u), e);
+  else
+    hipLaunchKernelGGL(init_kernel<double>, grid, dim3(256), 0, s, g, static_cast<double *>(u), e);
   return hipGetLastError();
 }
 
@@ -342,12 +353,16 @@ int norms_blocks(const DevGeom &g) {
   return (int)(b < 2048 ? b : 2048);
 }
 
-hipError_t launch_norms(const DevGeom &g, const double *u, double e, double *part_max,
+hipError_t launch_norms(const DevGeom &g, const void *u, bool f32, double e, double *part_max,
                         long long *part_idx, double *part_sq, int *nparts, hipStream_t s) {
   int nb = norms_blocks(g);
   *nparts = nb;
-  hipLaunchKernelGGL(norms_kernel, dim3(nb), dim3(256), 0, s, g, u, e, part_max, part_idx,
-                     part_sq);
+  if (f32)
+    hipLaunchKernelGGL(norms_kernel<float>, dim3(nb), dim3(256), 0, s, g,
+                       static_cast<const float *>(u), e, part_max, part_idx, part_sq);
+  else
+    hipLaunchKernelGGL(norms_kernel<double>, dim3(nb), dim3(256), 0, s, g,
+                       static_cast<const double *>(u), e, part_max, part_idx, part_sq);
   return hipGetLastError();
 }
 

@@ -10,12 +10,14 @@
 
 namespace mts {
 
-// One line stage along direction dir with the given flags.
-hipError_t launch_line(int dir, int flags, const LineArgs &a, hipStream_t s);
+// One line stage along direction dir with the given flags; f32: the fields
+// in LineArgs are FP32 (LOD and TS stages only).
+hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &a, hipStream_t s);
 
 // Per direction and lane-group size (line_kernels.hip); a.q already filled.
-#define MTS_DECL_DL(D, LL) \
-  hipError_t launch_line_dl_##D##_##LL(int flags, bool seq, const LineArgs &a, hipStream_t s);
+#define MTS_DECL_DL(D, LL)                                                              \
+  hipError_t launch_line_dl_##D##_##LL(int flags, bool seq, bool f32, const LineArgs &a, \
+                                       hipStream_t s);
 MTS_DECL_DL(0, 16) MTS_DECL_DL(0, 32) MTS_DECL_DL(0, 64)
 MTS_DECL_DL(1, 16) MTS_DECL_DL(1, 32) MTS_DECL_DL(1, 64)
 MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
@@ -27,16 +29,17 @@ hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s);
 // Douglas stage 1 at MIB-affected nodes (see device_core.hpp).
 hipError_t launch_spec(const SpecArgs &a, hipStream_t s);
 
-// Boundary faces: out = exact outer solution at e (+ cf * f(e) when cf != 0).
-hipError_t launch_faces(const DevGeom &g, const PhysConst &P, double *out, double e, double cf,
-                        hipStream_t s);
+// Boundary faces: out = exact outer solution at e (+ cf * f(e) when cf != 0);
+// f32: out is an FP32 field.
+hipError_t launch_faces(const DevGeom &g, const PhysConst &P, void *out, bool f32, double e,
+                        double cf, hipStream_t s);
 
 // Exact solution at e = exp(-t0) on every node (Initialization).
-hipError_t launch_init(const DevGeom &g, double *u, double e, hipStream_t s);
+hipError_t launch_init(const DevGeom &g, void *u, bool f32, double e, hipStream_t s);
 
 // Error norms against the exact solution: per-block partial (max, ref index,
-// sum of squares); nblocks written to *nparts.
-hipError_t launch_norms(const DevGeom &g, const double *u, double e, double *part_max,
+// sum of squares); nblocks written to *nparts.  Accumulated in FP64.
+hipError_t launch_norms(const DevGeom &g, const void *u, bool f32, double e, double *part_max,
                         long long *part_idx, double *part_sq, int *nparts, hipStream_t s);
 int norms_blocks(const DevGeom &g);
 

@@ -59,6 +59,41 @@ __device__ __forceinline__ double grp_down(double v, int s, int kseg, double id)
     return kseg + s >= L ? id : w;
   }
 }
+template <int L>
+__device__ __forceinline__ float grp_up(float v, int s, int kseg, float id) {
+  if (L == 16) {
+    const int ctl = 0x110 | s;
+    int w = 0;
+    switch (ctl) {
+#define MTS_DPP1(c)                                                                          \
+  case c:                                                                                    \
+    w = __builtin_amdgcn_update_dpp(__float_as_int(id), __float_as_int(v), c, 0xf, 0xf, false); \
+    break;
+      MTS_DPP1(0x111) MTS_DPP1(0x112) MTS_DPP1(0x114) MTS_DPP1(0x118)
+      default: break;
+    }
+    return __int_as_float(w);
+  } else {
+    const float w = __shfl_up(v, s, L);
+    return kseg < s ? id : w;
+  }
+}
+template <int L>
+__device__ __forceinline__ float grp_down(float v, int s, int kseg, float id) {
+  if (L == 16) {
+    const int ctl = 0x100 | s;
+    int w = 0;
+    switch (ctl) {
+      MTS_DPP1(0x101) MTS_DPP1(0x102) MTS_DPP1(0x104) MTS_DPP1(0x108)
+      default: break;
+    }
+    return __int_as_float(w);
+  } else {
+    const float w = __shfl_down(v, s, L);
+    return kseg + s >= L ? id : w;
+  }
+}
+#undef MTS_DPP1
 #undef MTS_DPP
 
 // 1 / x: hardware reciprocal refined by two Newton steps (within an ulp or
@@ -71,6 +106,15 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return fma(r, e, r);
 }
 
+// 1 / x in FP32: hardware reciprocal refined by one Newton step.
+__device__ __forceinline__ float rcp_nr(float x) {
+  const float r = __builtin_amdgcn_rcpf(x);
+  const float e = fmaf(-x, r, 1.0f);
+  return fmaf(r, e, r);
+}
+__device__ __forceinline__ double fmar(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return fmaf(a, b, c); }
+
 // Partitioned solve of the group's line: lane k holds rows [s_k, s_k + m)
 // of  a_i x_{i-1} + b_i x_i + c_i x_{i+1} = d_i  (m >= 2, or m == 0 on idle
 // lanes; the first row of the first segment has a = 0 and the last row of
@@ -79,15 +123,14 @@ __device__ __forceinline__ double rcp_nr(double x) {
 // out with the next lane's first row, the one-equation-per-lane system in
 // the segment-end unknowns is solved by parallel cyclic reduction over the
 // group, and each lane back-substitutes.  On return x holds the solution.
-template <int L, int MMAX>
-__device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)[MMAX],
-                                            double (&c)[MMAX], double (&d)[MMAX], int m,
-                                            int kseg) {
-  double A1 = 0.0, B1 = 1.0, C1 = 0.0, D1 = 0.0;
-  double A2 = 0.0, C2 = 0.0, D2 = 0.0;
+template <int L, int MMAX, typename R>
+__device__ __forceinline__ void group_solve(R (&a)[MMAX], const R (&b)[MMAX], R (&c)[MMAX],
+                                            R (&d)[MMAX], int m, int kseg) {
+  R A1 = 0, B1 = 1, C1 = 0, D1 = 0;
+  R A2 = 0, C2 = 0, D2 = 0;
   if (m >= 2) {
     {
-      const double r = rcp_nr(b[1]);
+      const R r = rcp_nr(b[1]);
       a[1] = a[1] * r;
       c[1] = c[1] * r;
       d[1] = d[1] * r;
@@ -98,8 +141,8 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
 #pragma unroll
     for (int i = 2; i < MMAX; i++) {
       if (i < m) {
-        const double r = rcp_nr(fma(-a[i], c[i - 1], b[i]));
-        d[i] = fma(-a[i], d[i - 1], d[i]) * r;
+        const R r = rcp_nr(fmar(-a[i], c[i - 1], b[i]));
+        d[i] = fmar(-a[i], d[i - 1], d[i]) * r;
         a[i] = -a[i] * a[i - 1] * r;
         c[i] = c[i] * r;
         A2 = a[i];
@@ -110,8 +153,8 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
 #pragma unroll
     for (int i = MMAX - 3; i >= 1; i--) {
       if (i <= m - 3) {
-        d[i] = fma(-c[i], d[i + 1], d[i]);
-        a[i] = fma(-c[i], a[i + 1], a[i]);
+        d[i] = fmar(-c[i], d[i + 1], d[i]);
+        a[i] = fmar(-c[i], a[i + 1], a[i]);
         c[i] = -c[i] * c[i + 1];
       }
     }
@@ -122,51 +165,51 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
       D1 = d[0];
     } else {
       A1 = a[0];
-      B1 = fma(-c[0], a[1], b[0]);
+      B1 = fmar(-c[0], a[1], b[0]);
       C1 = -c[0] * c[1];
-      D1 = fma(-c[0], d[1], d[0]);
+      D1 = fmar(-c[0], d[1], d[0]);
     }
   }
-  const double A1n = grp_down<L>(A1, 1, kseg, 0.0);
-  const double B1n = grp_down<L>(B1, 1, kseg, 1.0);
-  const double C1n = grp_down<L>(C1, 1, kseg, 0.0);
-  const double D1n = grp_down<L>(D1, 1, kseg, 0.0);
-  double ea = 0.0, eb = 1.0, ec = 0.0, ed = 0.0;
-  const double i1 = rcp_nr(B1);
+  const R A1n = grp_down<L>(A1, 1, kseg, R(0));
+  const R B1n = grp_down<L>(B1, 1, kseg, R(1));
+  const R C1n = grp_down<L>(C1, 1, kseg, R(0));
+  const R D1n = grp_down<L>(D1, 1, kseg, R(0));
+  R ea = 0, eb = 1, ec = 0, ed = 0;
+  const R i1 = rcp_nr(B1);
   if (m >= 2) {
-    const double in = rcp_nr(B1n);
+    const R in = rcp_nr(B1n);
     ea = -A2 * A1 * i1;
-    eb = 1.0 - A2 * C1 * i1 - C2 * A1n * in;
+    eb = R(1) - A2 * C1 * i1 - C2 * A1n * in;
     ec = -C2 * C1n * in;
     ed = D2 - A2 * D1 * i1 - C2 * D1n * in;
   }
 #pragma unroll
   for (int s = 1; s < L; s <<= 1) {
-    const double am = grp_up<L>(ea, s, kseg, 0.0), bm = grp_up<L>(eb, s, kseg, 1.0);
-    const double cm = grp_up<L>(ec, s, kseg, 0.0), dm = grp_up<L>(ed, s, kseg, 0.0);
-    const double ap = grp_down<L>(ea, s, kseg, 0.0), bp = grp_down<L>(eb, s, kseg, 1.0);
-    const double cp = grp_down<L>(ec, s, kseg, 0.0), dp = grp_down<L>(ed, s, kseg, 0.0);
-    const double k1 = ea * rcp_nr(bm), k2 = ec * rcp_nr(bp);
-    const double na = -am * k1;
-    const double nc = -cp * k2;
-    const double nb = fma(-ap, k2, fma(-cm, k1, eb));
-    const double nd = fma(-dp, k2, fma(-dm, k1, ed));
+    const R am = grp_up<L>(ea, s, kseg, R(0)), bm = grp_up<L>(eb, s, kseg, R(1));
+    const R cm = grp_up<L>(ec, s, kseg, R(0)), dm = grp_up<L>(ed, s, kseg, R(0));
+    const R ap = grp_down<L>(ea, s, kseg, R(0)), bp = grp_down<L>(eb, s, kseg, R(1));
+    const R cp = grp_down<L>(ec, s, kseg, R(0)), dp = grp_down<L>(ed, s, kseg, R(0));
+    const R k1 = ea * rcp_nr(bm), k2 = ec * rcp_nr(bp);
+    const R na = -am * k1;
+    const R nc = -cp * k2;
+    const R nb = fmar(-ap, k2, fmar(-cm, k1, eb));
+    const R nd = fmar(-dp, k2, fmar(-dm, k1, ed));
     ea = na;
     eb = nb;
     ec = nc;
     ed = nd;
   }
-  const double xe = ed * rcp_nr(eb);
-  const double xprev = grp_up<L>(xe, 1, kseg, 0.0);
+  const R xe = ed * rcp_nr(eb);
+  const R xprev = grp_up<L>(xe, 1, kseg, R(0));
   // (written as selects of computed values: conditional stores of one value
   // to several elements get merged into one indexed store, which keeps the
   // array in scratch memory)
-  const double xs = fma(-C1, xe, fma(-A1, xprev, D1)) * i1;
+  const R xs = fmar(-C1, xe, fmar(-A1, xprev, D1)) * i1;
   const bool act = m >= 2;
 #pragma unroll
   for (int i = 0; i < MMAX; i++) {
-    const double mid = fma(-c[i], xe, fma(-a[i], xs, d[i]));
-    double v = i == 0 ? xs : (i == m - 1 ? xe : mid);
+    const R mid = fmar(-c[i], xe, fmar(-a[i], xs, d[i]));
+    R v = i == 0 ? xs : (i == m - 1 ? xe : mid);
     d[i] = act && i < m ? v : d[i];
   }
 }
@@ -179,11 +222,11 @@ __device__ __forceinline__ void group_solve(double (&a)[MMAX], const double (&b)
 // gamma in a and the forward value, then the solution, in d.  Solved rows
 // become identity rows (a = c = 0, b = 1, d = x), which the partitioned
 // solve that follows returns unchanged.
-template <int L, int MMAX>
-__device__ __forceinline__ void group_thomas(double (&a)[MMAX], double (&b)[MMAX],
-                                             double (&c)[MMAX], double (&d)[MMAX],
-                                             const LinePart &q, int lane, int kseg,
+template <int L, int MMAX, typename R>
+__device__ __forceinline__ void group_thomas(R (&a)[MMAX], R (&b)[MMAX], R (&c)[MMAX],
+                                             R (&d)[MMAX], const LinePart &q, int lane, int kseg,
                                              bool seq_line, double x0, double xn) {
+  // in FP64 whatever R is: these lines are the ill-conditioned ones
   const int gb = lane - kseg;
   double bet = 1.0, uprev = x0, cprev = 0.0;
   for (int kk = 0; kk < q.p; kk++) {
@@ -192,14 +235,14 @@ __device__ __forceinline__ void group_thomas(double (&a)[MMAX], double (&b)[MMAX
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
       if (r < mk) {
-        const double ai = __shfl(a[r], src, 64), bi = __shfl(b[r], src, 64);
-        const double ci = __shfl(c[r], src, 64), di = __shfl(d[r], src, 64);
+        const double ai = __shfl((double)a[r], src, 64), bi = __shfl((double)b[r], src, 64);
+        const double ci = __shfl((double)c[r], src, 64), di = __shfl((double)d[r], src, 64);
         const double gam = cprev / bet;
         bet = bi - ai * gam;
         const double u = (di - ai * uprev) / bet;
         if (seq_line && kseg == kk) {
-          a[r] = gam;
-          d[r] = u;
+          a[r] = (R)gam;
+          d[r] = (R)u;
         }
         cprev = ci;
         uprev = u;
@@ -213,9 +256,9 @@ __device__ __forceinline__ void group_thomas(double (&a)[MMAX], double (&b)[MMAX
 #pragma unroll
     for (int r = MMAX - 1; r >= 0; r--) {
       if (r < mk) {
-        const double ui = __shfl(d[r], src, 64), gi = __shfl(a[r], src, 64);
+        const double ui = __shfl((double)d[r], src, 64), gi = __shfl((double)a[r], src, 64);
         const double u = ui - gnext * unext;
-        if (seq_line && kseg == kk) d[r] = u;
+        if (seq_line && kseg == kk) d[r] = (R)u;
         gnext = gi;
         unext = u;
       }
@@ -224,16 +267,31 @@ __device__ __forceinline__ void group_thomas(double (&a)[MMAX], double (&b)[MMAX
   if (seq_line) {
 #pragma unroll
     for (int r = 0; r < MMAX; r++) {
-      a[r] = 0.0;
-      b[r] = 1.0;
-      c[r] = 0.0;
+      a[r] = 0;
+      b[r] = 1;
+      c[r] = 0;
     }
   }
 }
 
 // ----------------------------------------------------------- line kernel
-template <int DIR, int L, int MMAX, int FLAGS>
+// Pair of neighbouring field elements moved as one access.
+template <typename R>
+struct Pair2;
+template <>
+struct Pair2<double> {
+  using type = double2;
+};
+template <>
+struct Pair2<float> {
+  using type = float2;
+};
+
+// R: the field type, double (FP64 form) or float (FP32 form: fields, LDS
+// tiles and the line solve in FP32, rows formed in FP64).
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
 __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
+  using V2 = typename Pair2<R>::type;
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
@@ -250,10 +308,13 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const unsigned magic_n = magic_of(n);
 
   extern __shared__ double smem[];
-  double *tA = smem;
-  double *tB = smem + T * q.lds_len;
+  R *tA = reinterpret_cast<R *>(smem);
+  R *tB = tA + T * q.lds_len;
+  const R *fA = static_cast<const R *>(p.A);
+  const R *fS = static_cast<const R *>(p.S);
+  R *fO = static_cast<R *>(p.out);
 
-  // stage the tile: 16-byte loads of element pairs, kChunk of them in flight
+  // stage the tile: loads of element pairs, kChunk of them in flight
   // per thread, then their LDS stores (chunks one after the other, so the
   // loads' registers stay bounded)
   constexpr int kChunk = 5;
@@ -261,15 +322,15 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const unsigned magic_nh = magic_of(nh);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
-    double2 va[kChunk], vb[kChunk];
+    V2 va[kChunk], vb[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
       const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
-      va[c] = make_double2(0.0, 0.0);
-      vb[c] = make_double2(0.0, 0.0);
+      va[c].x = va[c].y = R(0);
+      vb[c].x = vb[c].y = R(0);
       if (t.e0.ok) {
-        va[c] = *reinterpret_cast<const double2 *>(p.A + t.e0.gi);
-        if (HAS_S) vb[c] = *reinterpret_cast<const double2 *>(p.S + t.e0.gi);
+        va[c] = *reinterpret_cast<const V2 *>(fA + t.e0.gi);
+        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + t.e0.gi);
       }
     }
 #pragma unroll
@@ -301,27 +362,27 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     const bool valid = j >= 1 && j <= nj - 2;
     const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
     const LaneSeg sg = lane_seg(q, kseg, valid);
-    double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+    R a[MMAX], b[MMAX], c[MMAX], d[MMAX];
     line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len, sg, a, b, c,
                                 d);
     double x0 = 0.0, xn = 0.0;
     if (valid) line_ends<DIR>(p, j, k, x0, xn);
     if (SOLVE) {
       if (SEQ) {
-        if (__any(lm.seq)) group_thomas<L, MMAX>(a, b, c, d, q, lane, kseg, lm.seq, x0, xn);
+        if (__any(lm.seq)) group_thomas<L, MMAX, R>(a, b, c, d, q, lane, kseg, lm.seq, x0, xn);
       }
       // boundary rows are known: move their couplings to the right-hand side
       const bool first = sg.m > 0 && sg.k == 0;
-      d[0] = first ? d[0] - a[0] * x0 : d[0];
-      a[0] = first ? 0.0 : a[0];
+      d[0] = first ? (R)((double)d[0] - (double)a[0] * x0) : d[0];
+      a[0] = first ? R(0) : a[0];
       const bool lastseg = sg.m > 0 && sg.k == q.p - 1;
 #pragma unroll
       for (int r = 0; r < MMAX; r++) {
         const bool hit = lastseg && r == sg.m - 1;
-        d[r] = hit ? d[r] - c[r] * xn : d[r];
-        c[r] = hit ? 0.0 : c[r];
+        d[r] = hit ? (R)((double)d[r] - (double)c[r] * xn) : d[r];
+        c[r] = hit ? R(0) : c[r];
       }
-      group_solve<L, MMAX>(a, b, c, d, sg.m, kseg);
+      group_solve<L, MMAX, R>(a, b, c, d, sg.m, kseg);
     }
     if (valid)
       line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
@@ -329,15 +390,15 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   }
   __syncthreads();
 
-  const double *tO = TWO && !HAS_S ? tB : tA;
+  const R *tO = TWO && !HAS_S ? tB : tA;
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
-    double2 vo[kChunk];
+    V2 vo[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
       const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
-      vo[c].x = t.e0.ok ? tO[t.e0.sl] : 0.0;
-      vo[c].y = t.e1.ok ? tO[t.e1.sl] : 0.0;
+      vo[c].x = t.e0.ok ? tO[t.e0.sl] : R(0);
+      vo[c].y = t.e1.ok ? tO[t.e1.sl] : R(0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
@@ -345,34 +406,34 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
       const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
       const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
       if (w0 && w1) {
-        *reinterpret_cast<double2 *>(p.out + t.e0.gi) = vo[c];
+        *reinterpret_cast<V2 *>(fO + t.e0.gi) = vo[c];
       } else {
-        if (w0) p.out[t.e0.gi] = vo[c].x;
-        if (w1) p.out[t.e1.gi] = vo[c].y;
+        if (w0) fO[t.e0.gi] = vo[c].x;
+        if (w1) fO[t.e1.gi] = vo[c].y;
       }
     }
   }
 }
 
-template <int DIR, int L, int MMAX, int FLAGS>
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
 hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   const DevGeom &g = a.g;
   const int nj = DIR == 0 ? g.n[1] : g.n[0];
   const int nk = DIR == 2 ? g.n[1] : g.n[2];
   constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
   constexpr int T = tile_lines(L);
-  const size_t shm = (size_t)T * a.q.lds_len * sizeof(double) * (TWO ? 2 : 1);
+  const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1);
   dim3 grid((nj + T - 1) / T, nk - 2);
-  hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS>), grid, dim3(kBlock), shm, st, a);
+  hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), grid, dim3(kBlock), shm, st, a);
   return hipGetLastError();
 }
 
-template <int FLAGS>
+template <int FLAGS, typename R>
 hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
   constexpr int D = MTS_LINE_DIR, LL = MTS_LINE_L;
-  if (mmax == 4) return launch_line_t<D, LL, 4, FLAGS>(a, st);
-  if (mmax == 8) return launch_line_t<D, LL, 8, FLAGS>(a, st);
-  if (mmax == 16) return launch_line_t<D, LL, 16, FLAGS>(a, st);
+  if (mmax == 4) return launch_line_t<D, LL, 4, FLAGS, R>(a, st);
+  if (mmax == 8) return launch_line_t<D, LL, 8, FLAGS, R>(a, st);
+  if (mmax == 16) return launch_line_t<D, LL, 16, FLAGS, R>(a, st);
   return hipErrorInvalidValue;
 }
 
@@ -384,21 +445,33 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
 
 // The sequential-solve variant is compiled separately so that its register
 // needs do not lower the occupancy of the common case (no such line).
-#define MTS_LINE_CASE(F)                                                     \
+#define MTS_LINE_CASE(F, R)                                                  \
   case (F):                                                                  \
-    return seq ? launch_flags<(F) | kFlagSeq>(a.q.mmax, a, s)               \
-               : launch_flags<(F)>(a.q.mmax, a, s);
+    return seq ? launch_flags<(F) | kFlagSeq, R>(a.q.mmax, a, s)            \
+               : launch_flags<(F), R>(a.q.mmax, a, s);
 
-hipError_t MTS_ENTRY(int flags, bool seq, const LineArgs &a, hipStream_t s) {
+hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream_t s) {
+  if (f32) {
+    // the FP32 form: LOD and TS stages (Douglas-ADI runs in FP64 only)
+    switch (flags) {
+      MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
+      MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+      MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
+      MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+      MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
+      case kRhsCN: return launch_flags<kRhsCN, float>(a.q.mmax, a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   switch (flags) {
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve)
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump)
-    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF)
-    MTS_LINE_CASE(kRhsDouglas | kFlagSolve)
-    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump)
-    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF)
-    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump)
-    case kRhsCN: return launch_flags<kRhsCN>(a.q.mmax, a, s);
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve, double)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, double)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, double)
+    MTS_LINE_CASE(kRhsDouglas | kFlagSolve, double)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, double)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, double)
+    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, double)
+    case kRhsCN: return launch_flags<kRhsCN, double>(a.q.mmax, a, s);
     default: return hipErrorInvalidValue;
   }
 }

@@ -57,15 +57,17 @@ def test_request_and_result_layout_match_the_header(mts, tmp_path):
     src = tmp_path / "layout.c"
     src.write_text(
         '#include <stdio.h>\n#include <stddef.h>\n#include "mts.h"\n'
-        'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(mts_request), '
-        'offsetof(mts_request, tag), sizeof(mts_result), offsetof(mts_result, message), '
-        'offsetof(mts_result, argmax), offsetof(mts_result, footprint));return 0;}\n')
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu\\n", sizeof(mts_request), '
+        'offsetof(mts_request, tag), offsetof(mts_request, precision), sizeof(mts_result), '
+        'offsetof(mts_result, message), offsetof(mts_result, argmax), '
+        'offsetof(mts_result, footprint));return 0;}\n')
     exe = tmp_path / "layout"
     subprocess.run([cc, "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)],
                    check=True)
     got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True,
                                           text=True).stdout.split()]
-    want = [ctypes.sizeof(mts.Request), mts.Request.tag.offset, ctypes.sizeof(mts.Result),
+    want = [ctypes.sizeof(mts.Request), mts.Request.tag.offset, mts.Request.precision.offset,
+            ctypes.sizeof(mts.Result),
             mts.Result.message.offset, mts.Result.argmax.offset, mts.Result.footprint.offset]
     assert got == want
 

@@ -1,0 +1,103 @@
+"""The FP32 form (MTS_FP32): FP32 field storage, LDS tiles and line solve,
+rows formed in FP64 and rounded once, FP64 setup and norms.
+
+Bars, stated here because the form is not the reference's arithmetic:
+* per step, the FP32 field agrees with the FP64 field to 2e-5 of its
+  magnitude (FP32 rounding, 6e-8, amplified by the line systems' conditioning);
+* the L-infinity error against the exact solution stays within 1.05x of the
+  FP64 form's (the accuracy criterion the FP32 form is measured at).
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+
+CASES = [
+    # surface, args, beta, intervals, method, mib, dt, steps
+    ("ellipsoid", (0.7, 0.5, 0.4, 1 - 2 ** -20), ("sincos", 10, 1), 64, "lod-cn", 2, 1 / 32, 8),
+    ("ellipsoid", (0.7, 0.5, 0.4, 1 - 2 ** -20), ("sincos", 10, 1), 64, "lod-ie", 1, 1 / 32, 8),
+    ("torus", (0.5, 0.2), ("const", 0.01, 1), 48, "ts", 2, 1 / 48, 6),
+    ("bumpy", (0.5 * (1 - 2 ** -20), 0.1, 5, 3), ("const", 10, 1), 40, "lod-cn", 2, 1 / 40, 6),
+]
+
+
+def _pair(mts, ctx, case):
+    surf, sargs, beta, n, method, mib, dt, steps = case
+    st = mts.Setup(mts.grid(n + 1), mts.surface(surf, *sargs), mts.physics(*beta), mib=mib)
+    return (mts.Instance(ctx, st, method, dt, precision="fp64"),
+            mts.Instance(ctx, st, method, dt, precision="fp32"), st)
+
+
+@pytest.mark.parametrize("ci", range(len(CASES)))
+def test_fp32_follows_fp64_step_by_step(mts, ctx, ci):
+    case = CASES[ci]
+    print("fp32 case", case[0], case[3], case[4], "mib", case[5], flush=True)
+    i64, i32, _ = _pair(mts, ctx, case)
+    assert i32.precision == mts.FP32
+    i64.initialize(0.0)
+    i32.initialize(0.0)
+    u0 = i64.download()
+    assert rel_err(i32.download(), u0) <= 1e-7  # the initial field, rounded to FP32
+    t = 0.0
+    dt, steps = case[6], case[7]
+    for s in range(steps):
+        # restart the FP32 form from the FP64 field so each step is measured alone
+        i32.upload(i64.download())
+        t += dt
+        i64.step(t)
+        i32.step(t)
+        e = rel_err(i32.download(), i64.download())
+        print(f"  step {s + 1}: FP32 vs FP64 {e:.2e}", flush=True)
+        assert e <= 2e-5, f"step {s + 1}"
+    i64.close()
+    i32.close()
+
+
+@pytest.mark.parametrize("ci", [0, 1, 3])
+def test_fp32_error_within_5_percent_of_fp64(mts, ctx, ci):
+    case = CASES[ci]
+    i64, i32, _ = _pair(mts, ctx, case)
+    res = []
+    for inst in (i64, i32):
+        inst.initialize(0.0)
+        steps, t_end = inst.run(0.0, case[6] * case[7])
+        res.append(inst.error(t_end))
+        inst.close()
+    (l64, _, _), (l32, _, _) = res
+    print(f"  linf FP64 {l64:.6e} FP32 {l32:.6e}", flush=True)
+    assert l32 <= 1.05 * l64
+
+
+def test_fp32_douglas_is_refused(mts, ctx):
+    st = mts.Setup(mts.grid(33), mts.surface("ellipsoid", 0.6, 0.5, 0.4, 1 - 2 ** -20),
+                   mts.physics("const", 10, 1), mib=1)
+    with pytest.raises(mts.MtsError) as e:
+        mts.Instance(ctx, st, "adi", 1 / 16, precision="fp32")
+    assert e.value.code == -6
+
+
+def test_fp32_footprint_and_serving(mts, ctx):
+    """The FP32 field takes half the bytes; a served FP32 solve returns the
+    same norms as the same solve run alone."""
+    case = CASES[0]
+    i64, i32, _ = _pair(mts, ctx, case)
+    b64, b32 = i64.device_bytes(), i32.device_bytes()
+    nodes_padded = 80 * 65 * 65  # x pitch 80 for 65 nodes
+    assert b64 - b32 == nodes_padded * 4
+    i32.initialize(0.0)
+    steps, t_end = i32.run(0.0, case[6] * case[7])
+    alone = i32.error(t_end)
+    i64.close()
+    i32.close()
+    s = mts.Server(ctx, 8 << 30, max_inflight=4, setup_threads=2)
+    q = mts.request(mts.grid(case[3] + 1), mts.surface(case[0], *case[1]),
+                    mts.physics(*case[2]), case[4], case[6], case[6] * case[7], mib=case[5],
+                    tag=7, precision="fp32")
+    s.submit(q)
+    r = list(s.drain().values())[0]
+    s.close()
+    assert r["status"] == 0, r["message"]
+    assert (r["linf"], r["l2"], tuple(r["argmax"])) == alone
+    assert r["footprint"] == b32
