@@ -303,8 +303,22 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const LinePart &q = p.q;
   const int n = g.n[DIR];
   const int nj = LineMap<DIR>::nj(g);
-  const int j0 = blockIdx.x * T;
-  const int k = blockIdx.y + 1;
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (DIR != 0 && sizeof(R) == 4) {
+    // FP32 y/z tiles are 64 bytes wide, half of a 128-byte line: place the
+    // two tiles of each line on one XCD (workgroups go to the 8 XCDs in
+    // turn), dispatched one after the other, so the line is read from HBM
+    // once and the second half comes from that XCD's L2
+    const unsigned nb = gridDim.x * gridDim.y;
+    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+    unsigned v = b;
+    if (b < (nb & ~15u)) v = (((b >> 3) >> 1) * 8 + (b & 7)) * 2 + ((b >> 3) & 1);
+    bx = v % gridDim.x;
+    by = v / gridDim.x;
+  }
+  const int j0 = bx * T;
+  const int k = by + 1;
+  if (j0 >= nj) return;  // the launch rounds the FP32 y/z grid up to an even width
   const unsigned magic_n = magic_of(n);
 
   extern __shared__ double smem[];
@@ -423,7 +437,9 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
   constexpr int T = tile_lines(L);
   const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1);
-  dim3 grid((nj + T - 1) / T, nk - 2);
+  int gx = (nj + T - 1) / T;
+  if (DIR != 0 && sizeof(R) == 4) gx = (gx + 1) & ~1;  // tiles in pairs (see line_kernel)
+  dim3 grid(gx, nk - 2);
   hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), grid, dim3(kBlock), shm, st, a);
   return hipGetLastError();
 }
