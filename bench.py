@@ -129,6 +129,17 @@ def main():
     inst.profile(False)
     linf, l2, _ = inst.error(t)
 
+    # the same K steps as one captured graph (the first run captures it)
+    t_graph = (args.steps + 0.5) * dt
+    inst.initialize(0.0)
+    inst.run(0.0, t_graph, graph=True)
+    inst.initialize(0.0)
+    inst.sync()
+    t_start = time.perf_counter()
+    inst.run(0.0, t_graph, graph=True)
+    inst.sync()
+    graph_ms = (time.perf_counter() - t_start) / args.steps * 1e3
+
     nodes = n ** 3
     value = nodes * args.steps / elapsed
     # dominant stage: largest total time; algorithmic bytes = one FP64 read and
@@ -184,6 +195,7 @@ def main():
             "setup_s": setup_s,
             "device": dev_name,
             "stage_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in stages.items()},
+            "graph_ms_per_step": graph_ms,
         },
         "roofline": {
             "bound": "hbm",

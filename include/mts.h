@@ -177,6 +177,13 @@ int mts_step(mts_instance *inst, double t_next);
  * Returns the number of steps taken in *steps and the final time in *t_end. */
 int mts_run(mts_instance *inst, double t0, double t_finish, int *steps, double *t_end);
 
+/* mts_run with the whole time loop captured once as a HIP graph and launched
+ * as one unit (same kernels, same arguments, same results).  The graph is kept
+ * in the instance and replayed while t0 and t_finish stay the same.  Needs
+ * per-stage profiling off.  No counterpart in the reference: it removes the
+ * host's per-kernel launch cost, which bounds small grids. */
+int mts_run_graph(mts_instance *inst, double t0, double t_finish, int *steps, double *t_end);
+
 /* Wait for the instance's queued work. */
 int mts_sync(mts_instance *inst);
 
@@ -216,6 +223,11 @@ int mts_instance_device_bytes(const mts_instance *inst, size_t *bytes);
  * solve per process; replaces a loop of mts_run calls. */
 int mts_batch_run(mts_instance *const *insts, int count, const double *t0, const double *t_finish,
                   int *steps, double *t_end);
+
+/* mts_batch_run with each instance's time loop as one graph (mts_run_graph),
+ * the graphs launched back to back on the instances' streams. */
+int mts_batch_run_graph(mts_instance *const *insts, int count, const double *t0,
+                        const double *t_finish, int *steps, double *t_end);
 
 /* ------------------------------------------------------------- serving
  * A request loop with continuous batching: submitted solves get their host

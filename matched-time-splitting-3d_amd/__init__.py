@@ -143,6 +143,8 @@ def lib() -> C.CDLL:
             "mts_instance_precision": ([VP, P(I)], I),
             "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
             "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
+            "mts_batch_run_graph": ([VP, I, VP, VP, VP, VP], I),
+            "mts_run_graph": ([VP, D, D, P(I), P(D)], I),
             "mts_server_create": ([VP, C.c_size_t, I, I, P(VP)], I),
             "mts_server_submit": ([VP, P(Request), P(C.c_longlong)], I),
             "mts_server_poll": ([VP, VP, I, P(I)], I),
@@ -339,9 +341,12 @@ class Instance:
     def step(self, t_next: float):
         _check(lib().mts_step(self._h, t_next))
 
-    def run(self, t0: float, t_finish: float):
+    def run(self, t0: float, t_finish: float, graph: bool = False):
+        """The reference's time loop; graph=True captures it as one HIP graph
+        (kept and replayed for the same t0 and t_finish)."""
         steps, t_end = C.c_int(), C.c_double()
-        _check(lib().mts_run(self._h, t0, t_finish, C.byref(steps), C.byref(t_end)))
+        f = lib().mts_run_graph if graph else lib().mts_run
+        _check(f(self._h, t0, t_finish, C.byref(steps), C.byref(t_end)))
         return steps.value, t_end.value
 
     def sync(self):
@@ -385,16 +390,18 @@ def version() -> str:
     return lib().mts_version().decode()
 
 
-def batch_run(instances, t0, t_finish):
+def batch_run(instances, t0, t_finish, graph: bool = False):
     """Runs the time loops of several instances with their steps interleaved
-    (``mts_batch_run``); returns [(steps, t_end)] per instance."""
+    (``mts_batch_run``), or each loop as one graph (``mts_batch_run_graph``);
+    returns [(steps, t_end)] per instance."""
     n = len(instances)
     t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, np.float64), (n,)))
     tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_finish, np.float64), (n,)))
     hs = (C.c_void_p * n)(*[i._h.value for i in instances])
     steps = np.zeros(n, np.int32)
     t_end = np.zeros(n, np.float64)
-    _check(lib().mts_batch_run(hs, n, _dptr(t0), _dptr(tf), _dptr(steps), _dptr(t_end)))
+    f = lib().mts_batch_run_graph if graph else lib().mts_batch_run
+    _check(f(hs, n, _dptr(t0), _dptr(tf), _dptr(steps), _dptr(t_end)))
     return [(int(steps[i]), float(t_end[i])) for i in range(n)]
 
 

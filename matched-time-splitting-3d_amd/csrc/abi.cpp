@@ -179,8 +179,13 @@ struct mts_instance {
   std::map<std::string, int> stage_id;
   std::vector<double> stage_ms;
   std::vector<int> stage_n;
+  // the time loop captured as one graph (mts_run_graph), kept for repeated runs
+  hipGraphExec_t gexec = nullptr;
+  double g_t0 = 0.0, g_tf = -1.0, g_tend = 0.0;
+  int g_steps = 0;
   ~mts_instance() {
     if (stream) (void)hipStreamSynchronize(stream);  // pending copies may read host staging
+    if (gexec) (void)hipGraphExecDestroy(gexec);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
   size_t device_bytes() const {
@@ -889,6 +894,54 @@ int mts_run(mts_instance *in, double t0, double t_finish, int *steps, double *t_
   }
   if (steps) *steps = loop;
   if (t_end) *t_end = t;
+  return MTS_OK;
+}
+
+int mts_run_graph(mts_instance *in, double t0, double t_finish, int *steps, double *t_end) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  if (in->profile) return fail(MTS_E_STATE, "graph runs need per-stage profiling off");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  const int loop = (int)(t_finish / in->dt);
+  if (loop <= 0) {
+    if (steps) *steps = 0;
+    if (t_end) *t_end = t0;
+    return MTS_OK;
+  }
+  if (!in->gexec || in->g_t0 != t0 || in->g_tf != t_finish) {
+    if (in->gexec) {
+      MTS_HIP(hipStreamSynchronize(in->stream));
+      MTS_HIP(hipGraphExecDestroy(in->gexec));
+      in->gexec = nullptr;
+    }
+    // the steps are recorded exactly as mts_run issues them (kernels only)
+    MTS_HIP(hipStreamBeginCapture(in->stream, hipStreamCaptureModeThreadLocal));
+    double t = t0;
+    int rc = MTS_OK;
+    for (int j = 0; j < loop && rc == MTS_OK; j++) {
+      t += in->dt;
+      rc = mts_step(in, t);
+    }
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(in->stream, &graph);
+    if (rc != MTS_OK) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return rc;
+    }
+    if (ce != hipSuccess) return fail(MTS_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(ce));
+    const hipError_t ie = hipGraphInstantiate(&in->gexec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+      in->gexec = nullptr;
+      return fail(MTS_E_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    in->g_t0 = t0;
+    in->g_tf = t_finish;
+    in->g_steps = loop;
+    in->g_tend = t;
+  }
+  MTS_HIP(hipGraphLaunch(in->gexec, in->stream));
+  if (steps) *steps = in->g_steps;
+  if (t_end) *t_end = in->g_tend;
   return MTS_OK;
 }
 

@@ -131,6 +131,25 @@ extern "C" int mts_batch_run(mts_instance *const *insts, int count, const double
 //                    polls their events; it never blocks on the device.
 // Admission (arrival order, HBM cap, in-flight limit) runs under the lock
 // whenever a planned solve arrives or a finished one frees its memory.
+extern "C" int mts_batch_run_graph(mts_instance *const *insts, int count, const double *t0,
+                                   const double *t_finish, int *steps, double *t_end) {
+  if (count < 0 || (count > 0 && (!insts || !t0 || !t_finish)))
+    return mts::set_error(MTS_E_ARG, "null argument or negative count");
+  std::vector<int> st(count);
+  std::vector<double> te(count);
+  for (int i = 0; i < count; i++) {
+    const int rc = mts_run_graph(insts[i], t0[i], t_finish[i], &st[i], &te[i]);
+    if (rc) return rc;
+  }
+  for (int i = 0; i < count; i++) {
+    const int rc = mts_sync(insts[i]);
+    if (rc) return rc;
+    if (steps) steps[i] = st[i];
+    if (t_end) t_end[i] = te[i];
+  }
+  return MTS_OK;
+}
+
 struct mts_server {
   struct Pending {
     long long id;
