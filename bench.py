@@ -245,7 +245,7 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
         "value": value,
         "unit": "grid-pt-steps/s",
         "ms_per_step": elapsed / args.steps * 1e3,
-        "dtype": "f32 fields and solve, f64 rows",
+        "dtype": "f32 fields, rows and solve; f64 MIB rows, setup and norms",
         "linf_error": linf,
         "l2_error": l2,
         "linf_vs_fp64": linf / fp64_linf,
@@ -351,6 +351,16 @@ def bench_batch(args):
         t0 = time.perf_counter()
         out32 = mts.batch_run(insts32, 0.0, [d["T"] for d in ds])
         f32_s = time.perf_counter() - t0
+        # each solve's time loop as one graph: the first run captures, the second is timed
+        for inst in insts32:
+            inst.initialize(0.0)
+        mts.batch_run(insts32, 0.0, [d["T"] for d in ds], graph=True)
+        for inst in insts32:
+            inst.initialize(0.0)
+            inst.sync()
+        t0 = time.perf_counter()
+        mts.batch_run(insts32, 0.0, [d["T"] for d in ds], graph=True)
+        f32g_s = time.perf_counter() - t0
         ratios = []
         for inst, (steps, t_end), le in zip(insts32, out32, loop_err):
             l32 = inst.error(t_end)[0]
@@ -359,6 +369,8 @@ def bench_batch(args):
             inst.close()
         f32 = {"batched_s": f32_s, "value": work / f32_s, "speedup_vs_fp64_loop": loop_s / f32_s,
                "speedup_vs_fp64_batched": batch_s / f32_s,
+               "graph_batched_s": f32g_s, "graph_value": work / f32g_s,
+               "graph_speedup_vs_fp64_loop": loop_s / f32g_s,
                "linf_ratio_max": max(ratios) if ratios else None,
                "linf_ratio_median": float(np.median(ratios)) if ratios else None,
                "solves_compared": len(ratios)}

@@ -734,7 +734,6 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   const DevGeom &g = p.g;
   const LinePart &q = p.q;
   const double tdt = p.theta_dt;
-  const double mtdt = -tdt;
   const int s = sg.s, m = sg.m;
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
@@ -760,26 +759,30 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
       const bool out = ((cm >> r) & 1u) != 0;
       double vd[3];
       op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
-      a[r] = (R)(mtdt * vd[0]);
-      b[r] = (R)(1 - tdt * vd[1]);
-      c[r] = (R)(mtdt * vd[2]);
+      // regular rows in the field's arithmetic (FP64, or FP32 for the FP32
+      // form, whose rows are FP32 values either way)
+      const R w0 = (R)vd[0], w1 = (R)vd[1], w2 = (R)vd[2];
+      const R rt = (R)tdt;
+      a[r] = -rt * w0;
+      b[r] = R(1) - rt * w1;
+      c[r] = -rt * w2;
       if (RHS == kRhsCopy) {
         d[r] = la[base + r];
       } else {
         const R *f = RHS == kRhsDouglas ? lb : la;
-        const double fp = r == 0 ? f[sl_prev] : f[base + r - 1];
-        const double fc = f[base + r];
-        const double fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
-        const double sum = fp * vd[0] + fc * vd[1] + fn * vd[2];
+        const R fp = r == 0 ? f[sl_prev] : f[base + r - 1];
+        const R fc = f[base + r];
+        const R fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
+        const R sum = fp * w0 + fc * w1 + fn * w2;
         if (RHS == kRhsDouglas) {
-          d[r] = (R)(-p.dt * sum + (double)la[base + r]);
+          d[r] = -(R)p.dt * sum + la[base + r];
         } else {
-          double v = tdt * sum + fc;
+          R v = rt * sum + fc;
           if (RHS == kRhsTSZ) {
-            v = v + p.c_f1 * src.at(p.P, g, p.e_f1, out, j, k, i);
-            v = v + p.c_f2 * src.at(p.P, g, p.e_f2, out, j, k, i);
+            v = v + (R)(p.c_f1 * src.at(p.P, g, p.e_f1, out, j, k, i));
+            v = v + (R)(p.c_f2 * src.at(p.P, g, p.e_f2, out, j, k, i));
           }
-          d[r] = (R)v;
+          d[r] = v;
         }
       }
     }
