@@ -115,6 +115,20 @@ int mts_setup_coords(const mts_setup *s, int axis, double *out, size_t count);
 /* crossings of direction dir in line order (Intersections::ifpx/ifpy/ifpz) */
 int mts_setup_crossings(const mts_setup *s, int dir, mts_crossing *out, size_t count);
 
+/* End-of-run interface diagnostics at time t for the manufactured solution:
+ * fictitious-point errors fp = {Lmax, L2} (Refresh_Fp + Error_Fp,
+ * spatial/intersection.cc:1841-2281 and 4667-4771: each fictitious point from
+ * its weights, the exact solution and the exact jumps, relative to the other
+ * side's exact value; two per crossing) and jump errors jump = {Lmax, L2}
+ * (Refresh_Jump's jump.err and Error_Jump, :2294-2490 and 4594-4665: the GFM
+ * u_dir against the exact [beta u_d], relative).  Host FP64; the relative
+ * errors are infinite where the reference divides by an exact zero. */
+int mts_setup_diagnostics(const mts_setup *s, double t, double fp[2], double jump[2]);
+/* The same per crossing of direction dir (line order): the two fictitious
+ * points' errors and the signed jump error. */
+int mts_setup_crossing_errors(const mts_setup *s, int dir, double t, double *errl, double *errr,
+                              double *jerr, size_t count);
+
 /* Node classification alone, the Mesh constructor (mesh/mesh.cc:20-70):
  * +1 outside, -1 inside, MTS_LAYOUT_XFAST; MTS_E_GEOMETRY when the interface
  * passes through a node (where the reference exits). */

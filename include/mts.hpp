@@ -303,7 +303,7 @@ class Mesh {
 // Crossings and MIB weights (spatial/intersection.h:84-104), built by the
 // library once in FP64.  ifpx[iy][iz], ifpy[ix][iz], ifpz[ix][iy] list the
 // crossings of each line in order.  The jumps are analytic and evaluated by
-// the kernels at every step, so Refresh_Jump has nothing to refresh.
+// the kernels at every step; Refresh_Jump only sets the diagnostics' time.
 class Intersections {
  public:
   typedef std::vector<std::vector<std::vector<mts_crossing>>> Cubic_Intersection_Data;
@@ -328,7 +328,20 @@ class Intersections {
       for (const mts_crossing &x : c) (*out[d])[x.j][x.k].push_back(x);
     }
   }
-  void Refresh_Jump(Equation &, CubicDoub &, Beta &) {}
+  // The end-of-run diagnostics at the equation's time (intersection.h:95-98):
+  // Refresh_* take the time, Error_* print the reference's report blocks.
+  void Refresh_Fp(Equation &eq) { t_fp_ = eq.t(); }
+  void Refresh_Jump(Equation &eq, CubicDoub &, Beta &) { t_jump_ = eq.t(); }
+  void Error_Fp(std::ofstream &out) {
+    double fp[2], jp[2];
+    check(mts_setup_diagnostics(setup_.get(), t_fp_, fp, jp));
+    report(out, "------------------------ Error of FPs ------------------------", fp);
+  }
+  void Error_Jump(std::ofstream &out) {
+    double fp[2], jp[2];
+    check(mts_setup_diagnostics(setup_.get(), t_jump_, fp, jp));
+    report(out, "------------------------ Error of Jumps ------------------------", jp);
+  }
   const mts_setup *handle() const { return setup_.get(); }
   const mts_physics_desc &physics() const { return phys_; }
 
@@ -336,8 +349,16 @@ class Intersections {
   struct Del {
     void operator()(mts_setup *s) const { mts_setup_destroy(s); }
   };
+  static void report(std::ofstream &out, const char *title, const double v[2]) {
+    out << title << std::endl;
+    out << std::setprecision(3) << std::scientific;
+    out << "Lmax = " << v[0] << std::endl;
+    out << "L2   = " << v[1] << std::endl;
+    out << std::fixed << std::endl;
+  }
   std::unique_ptr<mts_setup, Del> setup_;
   mts_physics_desc phys_;
+  double t_fp_ = 0.0, t_jump_ = 0.0;
 };
 
 // ------------------------------------------------------ temporal solvers

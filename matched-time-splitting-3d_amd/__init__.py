@@ -121,6 +121,8 @@ def lib() -> C.CDLL:
             "mts_setup_indicator": ([VP, VP, C.c_size_t], I),
             "mts_setup_coords": ([VP, I, VP, C.c_size_t], I),
             "mts_setup_crossings": ([VP, I, VP, C.c_size_t], I),
+            "mts_setup_diagnostics": ([VP, D, VP, VP], I),
+            "mts_setup_crossing_errors": ([VP, I, D, VP, VP, VP, C.c_size_t], I),
             "mts_ctx_create": ([I, P(VP)], I),
             "mts_ctx_destroy": ([VP], None),
             "mts_ctx_device_info": ([VP, C.c_char_p, C.c_size_t, P(C.c_size_t)], I),
@@ -253,6 +255,22 @@ class Setup:
         out = np.empty(n, dtype=np.float64)
         _check(lib().mts_setup_coords(self._h, axis, _dptr(out), n))
         return out
+
+    def diagnostics(self, t: float):
+        """((FP Lmax, FP L2), (jump Lmax, jump L2)) at time t (Error_Fp, Error_Jump)."""
+        fp = np.zeros(2)
+        jp = np.zeros(2)
+        _check(lib().mts_setup_diagnostics(self.handle, t, _dptr(fp), _dptr(jp)))
+        return (float(fp[0]), float(fp[1])), (float(jp[0]), float(jp[1]))
+
+    def crossing_errors(self, d: int, t: float):
+        """Per crossing of direction d: (errl, errr, jump error) arrays."""
+        n = int(self.info().n_cross[d])
+        el, er, je = np.zeros(n), np.zeros(n), np.zeros(n)
+        if n:
+            _check(lib().mts_setup_crossing_errors(self.handle, d, t, _dptr(el), _dptr(er),
+                                                   _dptr(je), n))
+        return el, er, je
 
     def crossings(self, d: int) -> np.ndarray:
         n = int(self.info().n_cross[d])

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -15,6 +16,7 @@
 
 #include "abi_internal.hpp"
 #include "device_core.hpp"
+#include "diagnostics.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
 #include "kernels.hpp"
@@ -491,6 +493,25 @@ int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
 }
 
 void mts_setup_destroy(mts_setup *s) { delete s; }
+
+int mts_setup_diagnostics(const mts_setup *s, double t, double fp[2], double jump[2]) {
+  if (!s || !fp || !jump) return fail(MTS_E_ARG, "null argument");
+  mts::interface_diagnostics(s->res, s->beta, t, fp, jump);
+  return MTS_OK;
+}
+
+int mts_setup_crossing_errors(const mts_setup *s, int dir, double t, double *errl, double *errr,
+                              double *jerr, size_t count) {
+  if (!s || !errl || !errr || !jerr) return fail(MTS_E_ARG, "null argument");
+  if (dir < 0 || dir > 2) return fail(MTS_E_ARG, "dir must be 0, 1 or 2");
+  if (count < s->res.cross[dir].size()) return fail(MTS_E_ARG, "output smaller than the crossing count");
+  std::vector<double> l, r, j;
+  mts::crossing_errors(s->res, s->beta, dir, t, l, r, j);
+  std::copy(l.begin(), l.end(), errl);
+  std::copy(r.begin(), r.end(), errr);
+  std::copy(j.begin(), j.end(), jerr);
+  return MTS_OK;
+}
 
 int mts_mesh_classify(const mts_grid_desc *grid, const mts_surface_desc *surface, int8_t *out,
                       size_t count) {

@@ -2,7 +2,8 @@
 // reference's main (main.cc:26-133) with its command line (-i input -o
 // output, appended to), configuration keys (configuration/read_config.cc),
 // configuration block (configuration/write_config.h:94-127), time loops and
-// error report (helper.h:199-427, douglas_adi.cc:1723-1770).
+// error report with the interface diagnostics (helper.h:199-427,
+// intersection.cc:4594-4771, douglas_adi.cc:1723-1770).
 //
 // Values this build evaluates on the GPU:
 //   surface   = ellipsoid | torus                 the reference's default
@@ -270,11 +271,17 @@ int main(int argc, char *argv[]) {
   // The time loops of helper.h:199-427: loop = (int)(t_finish / dt), t += dt.
   const int loop = (int)(c.t1 / c.dt);
   double t_now = c.t0;
+  // the report of helper.h:222-236: time, fictitious-point and jump errors,
+  // then the solution's error
   auto report = [&](detail::Solver &s, const char *title) {
     out << title << std::endl << std::endl;
     out << std::setprecision(1) << std::scientific << "T = " << t_now << std::endl;
     out << std::fixed;
     Equation eq(t_now, beta);
+    inter.Refresh_Fp(eq);
+    inter.Error_Fp(out);
+    inter.Refresh_Jump(eq, uh, beta);
+    inter.Error_Jump(out);
     s.Error(eq, uh, out);
   };
   if (c.method == "adi") {

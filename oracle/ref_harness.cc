@@ -448,6 +448,24 @@ int main(int argc, char **argv) {
     ncross[d.dir - 'x']++;
   });
 
+  // end-of-run diagnostics at the final time, as the reference's driver
+  // reports them (helper.h:222-236): fictitious-point errors (Refresh_Fp,
+  // intersection.cc:1841) and jump errors (Refresh_Jump), per crossing, plus
+  // the reference's own Error_Fp / Error_Jump report text
+  if (o.dump) {
+    inter.Refresh_Fp(eqT);
+    inter.Refresh_Jump(eqT, uh, *beta);
+    FILE *fd = fopen((o.out + "/diag.txt").c_str(), "w");
+    for_each_crossing(inter, nx, ny, nz, [&](Intersection_Data &d, int, int) {
+      fprintf(fd, "%.17g %.17g %.17g\n", d.err.errl.empty() ? 0.0 : d.err.errl[0],
+              d.err.errr.empty() ? 0.0 : d.err.errr[0], d.jump.err);
+    });
+    fclose(fd);
+    ofstream rep(o.out + "/diag_report.txt");
+    inter.Error_Fp(rep);
+    inter.Error_Jump(rep);
+  }
+
   FILE *f = fopen((o.out + "/summary.txt").c_str(), "w");
   fprintf(f, "nx %d\nny %d\nnz %d\nsteps %d\nt_final %.17g\n", nx, ny, nz, loop, t_now);
   fprintf(f, "lmax %.17g\nl2 %.17g\nargmax %d %d %d\n", lmax, l2, xm, ym, zm);

@@ -15,6 +15,11 @@ Each case becomes tests/golden/<name>.npz holding
   summary      steps, t_final, lmax, l2, argmax(3)
   jumps        [u] and u_dir per crossing after step 1's Refresh_Jump, in the
                reference's dump order (z lines, y lines, x lines)
+  diag{d}      per crossing of direction d, in the product's line order: the
+               relative errors of its two fictitious points and of its jump
+               u_dir at the final time (Refresh_Fp, Refresh_Jump)
+  diag_report  the reference's Error_Fp and Error_Jump lines (FP Lmax, L2,
+               jump Lmax, L2; printed with 3 digits)
 The vectors are data produced by the reference binary; no reference source is
 stored here.
 """
@@ -78,6 +83,21 @@ def main():
             continue
         nn = case["n"] + 1
         arrays = {"meta": np.array(json.dumps(case))}
+        diag_path = os.path.join(out, "diag.txt")
+        diag = np.loadtxt(diag_path, ndmin=2) if os.path.getsize(diag_path) else np.zeros((0, 3))
+        # the dump runs over z lines, then y lines, then x lines
+        counts = [len(ref["crossings"][d]) for d in range(3)]
+        starts = {2: 0, 1: counts[2], 0: counts[2] + counts[1]}
+        for d in range(3):
+            cr0 = ref["crossings"][d]
+            dd0 = diag[starts[d]:starts[d] + counts[d]]
+            if len(cr0):
+                order = np.argsort(cr0["k"].astype(np.int64) * nn + cr0["j"], kind="stable")
+                dd0 = dd0[order]
+            arrays[f"diag{d}"] = dd0.astype(np.float64)
+        rep = open(os.path.join(out, "diag_report.txt")).read().split()
+        nums = [float(rep[i + 2]) for i, w in enumerate(rep) if w in ("Lmax", "L2") and rep[i + 1] == "="]
+        arrays["diag_report"] = np.array(nums, dtype=np.float64)
         for d in range(3):
             nj = nn  # all axes have n + 1 nodes
             cr = orc.sort_by_line(ref["crossings"][d], nj)

@@ -50,6 +50,10 @@ def test_driver_reproduces_the_reference_report(tmp_path):
     assert p.returncode == 0, p.stdout + p.stderr
     text = out.read_text()
     assert "Error of Douglas-ADI with MIB" in text
+    # the interface diagnostics, as the reference's report lines
+    rep = [f"{v:.3e}" for v in g["diag_report"]]
+    assert f"Error of FPs ------------------------\nLmax = {rep[0]}\nL2   = {rep[1]}" in text
+    assert f"Error of Jumps ------------------------\nLmax = {rep[2]}\nL2   = {rep[3]}" in text
     steps, t_end, linf, l2, ax, ay, az = g["summary"][:7]
     m = re.search(r"T = (\S+)", text)
     assert float(m.group(1)) == float(f"{t_end:.1e}")
