@@ -321,7 +321,9 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       if (!mts::make_line_part(g.n[d], q, p->min_lanes))
         return fail(MTS_E_ARG, "line length above 1024 nodes is not supported");
     }
-    p->px = (g.n[0] + 15) / 16 * 16;
+    // x rows start on 128-byte lines: 16 doubles, 32 floats
+    const int align = precision == MTS_FP32 ? 32 : 16;
+    p->px = (g.n[0] + align - 1) / align * align;
     p->field_elems = (size_t)p->px * g.n[1] * g.n[2];
     mts::DeviceTableHost &t = p->t;
     const std::string terr =

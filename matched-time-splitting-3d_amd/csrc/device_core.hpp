@@ -461,7 +461,7 @@ constexpr int kTileElsPerThread = 17;
 // neighbouring rows of one line (x lines, each line taken as ceil(n/2)
 // pairs).  The second element of a pair may not exist (a line past the grid,
 // the row past an odd line's end); the 16 bytes are then still inside the
-// padded x row (the pitch is a multiple of 16 doubles and above n when n is
+// padded x row (the pitch is a whole number of 128-byte lines, above n when n is
 // odd), so the access stays inside the field.
 struct TilePair {
   TileAt e0, e1;

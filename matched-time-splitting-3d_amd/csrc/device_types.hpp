@@ -71,7 +71,8 @@ struct PhysConst {
 };
 
 // Device view of the grid: sizes, x pitch (row stride of the field arrays,
-// a multiple of 16 doubles so every x row starts on a 128-byte line),
+// 128 bytes times a whole number (16 doubles, 32 floats) so every x row starts
+// on a 128-byte line),
 // node classification and the per-axis sin/cos tables.
 struct DevGeom {
   int n[3];

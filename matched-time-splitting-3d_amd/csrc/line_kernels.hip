@@ -275,6 +275,13 @@ __device__ __forceinline__ void group_thomas(R (&a)[MMAX], R (&b)[MMAX], R (&c)[
 }
 
 // ----------------------------------------------------------- line kernel
+// Tiles that share one 128-byte line: y/z tiles are T consecutive x, so a
+// tile narrower than 128 bytes is one of 128 / (T * sizeof(R)) per line.
+template <int DIR, int T, typename R>
+__device__ __host__ constexpr unsigned tile_group() {
+  return (DIR != 0 && T * sizeof(R) < 128) ? 128u / (T * sizeof(R)) : 1u;
+}
+
 // Pair of neighbouring field elements moved as one access.
 template <typename R>
 struct Pair2;
@@ -304,21 +311,22 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const int n = g.n[DIR];
   const int nj = LineMap<DIR>::nj(g);
   int bx = blockIdx.x, by = blockIdx.y;
-  if (DIR != 0 && sizeof(R) == 4) {
-    // FP32 y/z tiles are 64 bytes wide, half of a 128-byte line: place the
-    // two tiles of each line on one XCD (workgroups go to the 8 XCDs in
+  constexpr unsigned G = tile_group<DIR, T, R>();
+  if (G > 1) {
+    // y/z tiles narrower than a 128-byte line (T * sizeof(R) < 128): place
+    // the G tiles of each line on one XCD (workgroups go to the 8 XCDs in
     // turn), dispatched one after the other, so the line is read from HBM
-    // once and the second half comes from that XCD's L2
+    // once and the other parts come from that XCD's L2
     const unsigned nb = gridDim.x * gridDim.y;
     const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
     unsigned v = b;
-    if (b < (nb & ~15u)) v = (((b >> 3) >> 1) * 8 + (b & 7)) * 2 + ((b >> 3) & 1);
+    if (b < nb / (8 * G) * (8 * G)) v = ((b >> 3) / G * 8 + (b & 7)) * G + (b >> 3) % G;
     bx = v % gridDim.x;
     by = v / gridDim.x;
   }
   const int j0 = bx * T;
   const int k = by + 1;
-  if (j0 >= nj) return;  // the launch rounds the FP32 y/z grid up to an even width
+  if (j0 >= nj) return;  // the launch rounds the grid width up to a multiple of G
   const unsigned magic_n = magic_of(n);
 
   extern __shared__ double smem[];
@@ -438,7 +446,8 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   constexpr int T = tile_lines(L);
   const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1);
   int gx = (nj + T - 1) / T;
-  if (DIR != 0 && sizeof(R) == 4) gx = (gx + 1) & ~1;  // tiles in pairs (see line_kernel)
+  constexpr int G = (int)tile_group<DIR, T, R>();
+  gx = (gx + G - 1) / G * G;  // tiles in groups of G (see line_kernel)
   dim3 grid(gx, nk - 2);
   hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), grid, dim3(kBlock), shm, st, a);
   return hipGetLastError();

@@ -86,8 +86,9 @@ def test_fp32_footprint_and_serving(mts, ctx):
     case = CASES[0]
     i64, i32, _ = _pair(mts, ctx, case)
     b64, b32 = i64.device_bytes(), i32.device_bytes()
-    nodes_padded = 80 * 65 * 65  # x pitch 80 for 65 nodes
-    assert b64 - b32 == nodes_padded * 4
+    # x rows padded to 128 bytes: pitch 80 doubles or 96 floats for 65 nodes;
+    # field plus the node classification byte, the rest the same
+    assert b64 - 80 * 65 * 65 * 9 == b32 - 96 * 65 * 65 * 5
     i32.initialize(0.0)
     steps, t_end = i32.run(0.0, case[6] * case[7])
     alone = i32.error(t_end)
