@@ -441,21 +441,24 @@ class Server:
                  setup_threads: int = 4):
         self._h = C.c_void_p()
         self.ctx = ctx
-        self._keep = []  # requests stay referenced while the loop may read them
+        self._keep = {}  # id -> request, kept until its result is polled (callback surfaces)
         _check(lib().mts_server_create(ctx.handle, int(hbm_cap_bytes), int(max_inflight),
                                        int(setup_threads), C.byref(self._h)))
 
     def submit(self, req: Request) -> int:
         rid = C.c_longlong()
-        self._keep.append(req)
         _check(lib().mts_server_submit(self._h, C.byref(req), C.byref(rid)))
+        self._keep[rid.value] = req
         return rid.value
 
     def poll(self, capacity: int = 256):
         buf = (Result * capacity)()
         cnt = C.c_int()
         _check(lib().mts_server_poll(self._h, buf, capacity, C.byref(cnt)))
-        return [buf[i].as_dict() for i in range(cnt.value)]
+        out = [buf[i].as_dict() for i in range(cnt.value)]
+        for r in out:
+            self._keep.pop(r["id"], None)
+        return out
 
     def wait(self, max_outstanding: int = 0):
         _check(lib().mts_server_wait(self._h, int(max_outstanding)))

@@ -123,14 +123,6 @@ extern "C" int mts_batch_run(mts_instance *const *insts, int count, const double
   return MTS_OK;
 }
 
-// The serving loop runs three kinds of threads:
-//   setup threads    Mesh + Intersections, MIB tables and stage records on the host;
-//   worker threads   create an admitted solve's instance (allocate, upload,
-//                    initialize) and finish a completed one (error norms, free);
-//   stepping thread  issues one step per round for every solve in flight and
-//                    polls their events; it never blocks on the device.
-// Admission (arrival order, HBM cap, in-flight limit) runs under the lock
-// whenever a planned solve arrives or a finished one frees its memory.
 extern "C" int mts_batch_run_graph(mts_instance *const *insts, int count, const double *t0,
                                    const double *t_finish, int *steps, double *t_end) {
   if (count < 0 || (count > 0 && (!insts || !t0 || !t_finish)))
@@ -150,6 +142,14 @@ extern "C" int mts_batch_run_graph(mts_instance *const *insts, int count, const 
   return MTS_OK;
 }
 
+// The serving loop runs three kinds of threads:
+//   setup threads    Mesh + Intersections, MIB tables and stage records on the host;
+//   worker threads   create an admitted solve's instance (allocate, upload,
+//                    initialize) and finish a completed one (error norms, free);
+//   stepping thread  issues one step per round for every solve in flight and
+//                    polls their events; it never blocks on the device.
+// Admission (arrival order, HBM cap, in-flight limit) runs under the lock
+// whenever a planned solve arrives or a finished one frees its memory.
 struct mts_server {
   struct Pending {
     long long id;
