@@ -71,3 +71,36 @@ def test_driver_reproduces_the_reference_report(tmp_path):
     m = re.search(r"Lmax = (\S+);  L2 = (\S+)", text)
     assert m.group(1) == f"{linf:.3e}" and m.group(2) == f"{l2:.3e}"
     assert "CPU time cost" in text
+
+
+@pytest.mark.parametrize("name", ["sphere16_lod-cn_mib2", "sphere16_lod-ie_mib1", "sphere16_ts_mib1",
+                                  "torus22_lodcn_mib2"])
+def test_driver_other_methods_match_the_reference(tmp_path, name):
+    """The driver's LOD and TS branches on golden cases: the reported L-inf
+    error and the interface diagnostics at the reference's printed precision."""
+    g = load_golden(name)
+    meta = g["meta"]
+    s = meta["surface"]
+    beta = meta["beta"]
+    cfg = tmp_path / "config.txt"
+    surf = (f"ellipsoid({s[1]!r},{s[2]!r},{s[3]!r},{s[4]!r})" if s[0] == "ellipsoid"
+            else f"torus({s[1]!r},{s[2]!r})")
+    cfg.write_text("\n".join([
+        "x-max = 1", "x-min = -1", "y-max = 1", "y-min = -1", "z-max = 1", "z-min = -1",
+        f"x-grid = {meta['n'] + 1}", f"y-grid = {meta['n'] + 1}", f"z-grid = {meta['n'] + 1}",
+        "time-start = 0", f"time-finish = {meta['T']!r}", f"time-step = {meta['dt']!r}",
+        f"surface = {surf}", f"temporal-method = {meta['method']}", "equation = exp-sin-cos",
+        f"beta = {beta[0]}({beta[1]},{beta[2]})", f"spatial-method = mib-v{meta['mib']}",
+        "spatial-accuracy = 2", ""]))
+    out = tmp_path / "out.txt"
+    print("mts_driver", name, flush=True)
+    p = subprocess.run([os.path.join(BUILD, "mts_driver"), "-i", str(cfg), "-o", str(out)],
+                       capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout + p.stderr
+    text = out.read_text()
+    linf = float(g["summary"][2])
+    m = re.search(r"Lmax = (\S+);  L2 = (\S+)", text)
+    assert abs(float(m.group(1)) - linf) <= 5e-4 * abs(linf)  # printed with 3 digits
+    rep = [f"{v:.3e}" for v in g["diag_report"]]
+    assert f"Error of FPs ------------------------\nLmax = {rep[0]}\nL2   = {rep[1]}" in text
+    assert f"Error of Jumps ------------------------\nLmax = {rep[2]}\nL2   = {rep[3]}" in text
