@@ -103,7 +103,10 @@ const char *mts_version(void);
  * (Mesh::Mesh mesh/mesh.cc:20-70), crossings, normals, local frames and the
  * MIB-L1/L2 fictitious-point weights (Intersections::Intersections
  * spatial/intersection.cc:22-51 and its setup chain :79-1830).
- * mib = 1 (MIB-L1, "mib-v1") or 2 (MIB-L2, "mib-v2"); accuracy = 2. */
+ * mib = 1 (MIB-L1, "mib-v1") or 2 (MIB-L2, "mib-v2"); accuracy = 2.
+ * The work splits over host threads (MTS_SETUP_THREADS, default the hardware
+ * threads up to 16; one for CALLBACK surfaces, whose callbacks are never
+ * called concurrently); the result is the same on any thread count. */
 int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
                      const mts_physics_desc *physics, int mib, int accuracy, mts_setup **out);
 void mts_setup_destroy(mts_setup *s);

@@ -3,9 +3,12 @@
 #include "host_setup.hpp"
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <string>
+
+#include "host_parallel.hpp"
 
 namespace mts {
 
@@ -365,11 +368,16 @@ void local_frame(int d, const double a[3], double p[3][3]) {
   }
 }
 
+// Crossings of a range of lines: reads the classification, appends to its
+// own crossing list (one Builder per chunk of lines).
 struct Builder {
   const Grid &g;
   const Surface &s;
   const BetaModel &beta;
-  SetupResult &out;
+  const SetupResult &out;          // mib and ind
+  std::vector<Crossing> &cross;    // this chunk's crossings, in line order
+  int singular_warnings = 0;
+  int corner_pairs = 0;
   std::string err;
 
   // Geometry of one crossing in the cell (i, i+1) of line (d, j, k).
@@ -406,7 +414,7 @@ struct Builder {
   }
 
   void warn_if_singular(const DenseLU &lu) {
-    if (std::fabs(lu.det()) < kTolSetup) out.singular_warnings++;
+    if (std::fabs(lu.det()) < kTolSetup) singular_warnings++;
   }
 
   // MIB-L1 pair of fictitious points from 2-point interpolations
@@ -638,7 +646,7 @@ struct Builder {
           if (!fill(d, j, k, i, count, c)) return false;
           if (out.mib == 1) mib_l1_irregular(c, g.h[d]);
           else mib_l2_irregular(c, g.h[d]);
-          out.cross[d].push_back(c);
+          cross.push_back(c);
           i += 2;
         } else {
           if (i + 3 > n - 1) {
@@ -656,9 +664,9 @@ struct Builder {
           if (!fill(d, j, k, i + 1, -count, R)) return false;
           if (out.mib == 1) mib_l1_corner(L, R, g.h[d]);
           else mib_l2_corner(L, R, g.h[d]);
-          out.cross[d].push_back(L);
-          out.cross[d].push_back(R);
-          out.corner_pairs[d]++;
+          cross.push_back(L);
+          cross.push_back(R);
+          corner_pairs++;
           i += 3;
         }
       } else {
@@ -676,22 +684,33 @@ struct Builder {
 }  // namespace
 
 std::string classify_nodes(const Grid &g, const Surface &s, std::vector<int8_t> &ind) {
-  // mesh.cc:51-69: set_node > 1e-12 outside, < 1e-12 inside, equal: exit
+  // mesh.cc:51-69: set_node > 1e-12 outside, < 1e-12 inside, equal: exit.
+  // Planes of z in parallel; the error named is the first node in the
+  // reference's order that lies on the interface.
   ind.resize(g.nodes());
-  for (int iz = 0; iz < g.n[2]; iz++)
-    for (int iy = 0; iy < g.n[1]; iy++)
-      for (int ix = 0; ix < g.n[0]; ix++) {
-        double v = s.set_node(g.c[0][ix], g.c[1][iy], g.c[2][iz]);
-        int8_t t;
-        if (v > kTolSetup) t = 1;
-        else if (v < kTolSetup) t = -1;
-        else {
-          char buf[128];
-          std::snprintf(buf, sizeof(buf), "interface on grid node (%d, %d, %d)", ix, iy, iz);
-          return buf;
+  const int nz = g.n[2];
+  const int threads = s.concurrent() ? setup_threads() : 1;
+  const int chunks = chunk_count(nz, threads);
+  std::vector<std::string> errs((size_t)chunks);
+  parallel_chunks(nz, chunks, threads, [&](int c, int z0, int z1) {
+    for (int iz = z0; iz < z1; iz++)
+      for (int iy = 0; iy < g.n[1]; iy++)
+        for (int ix = 0; ix < g.n[0]; ix++) {
+          double v = s.set_node(g.c[0][ix], g.c[1][iy], g.c[2][iz]);
+          int8_t t;
+          if (v > kTolSetup) t = 1;
+          else if (v < kTolSetup) t = -1;
+          else {
+            char buf[128];
+            std::snprintf(buf, sizeof(buf), "interface on grid node (%d, %d, %d)", ix, iy, iz);
+            errs[(size_t)c] = buf;
+            return;
+          }
+          ind[g.idx(ix, iy, iz)] = t;
         }
-        ind[g.idx(ix, iy, iz)] = t;
-      }
+  });
+  for (auto &e : errs)
+    if (!e.empty()) return e;
   return std::string();
 }
 
@@ -709,20 +728,46 @@ std::string build_setup(const Grid &g, const Surface &s, const BetaModel &beta,
     std::string err = classify_nodes(g, s, out.ind);
     if (!err.empty()) return err;
   }
-  Builder b{g, s, beta, out, std::string()};
+  // Lines of one direction in chunks of rows k, each chunk its own crossing
+  // list; stitched in line order, so crossings, line_ptr and the error named
+  // (the first line in the reference's scan order) match a one-thread scan.
+  const int threads = s.concurrent() ? setup_threads() : 1;
   for (int d = 0; d < 3; d++) {
     int ja, ka;
     line_axes(d, ja, ka);
     int nj = g.n[ja], nk = g.n[ka];
     out.n_lines[d] = nj * nk;
     out.line_ptr[d].assign((size_t)nj * nk + 1, 0);
-    for (int k = 0; k < nk; k++)
-      for (int j = 0; j < nj; j++) {
-        int lid = k * nj + j;
-        out.line_ptr[d][lid] = (int)out.cross[d].size();
-        if (j == 0 || k == 0 || j == nj - 1 || k == nk - 1) continue;
-        if (!b.scan_line(d, j, k)) return b.err;
-      }
+    const int chunks = chunk_count(nk, threads);
+    std::vector<std::vector<Crossing>> part((size_t)chunks);
+    std::vector<Builder> bs;
+    bs.reserve((size_t)chunks);
+    for (int c = 0; c < chunks; c++) bs.push_back(Builder{g, s, beta, out, part[(size_t)c], 0, 0, std::string()});
+    parallel_chunks(nk, chunks, threads, [&](int c, int k0, int k1) {
+      Builder &b = bs[(size_t)c];
+      for (int k = k0; k < k1; k++)
+        for (int j = 0; j < nj; j++) {
+          int lid = k * nj + j;
+          out.line_ptr[d][lid] = (int)b.cross.size();  // chunk-local until stitched
+          if (j == 0 || k == 0 || j == nj - 1 || k == nk - 1) continue;
+          if (!b.scan_line(d, j, k)) return;
+        }
+    });
+    for (auto &b : bs)
+      if (!b.err.empty()) return b.err;
+    size_t total = 0;
+    for (auto &pc : part) total += pc.size();
+    if (total > (size_t)INT32_MAX) return "too many interface crossings";
+    out.cross[d].reserve(total);
+    for (int c = 0; c < chunks; c++) {
+      const int base = (int)out.cross[d].size();
+      const int l0 = chunk_begin(nk, chunks, c) * nj, l1 = chunk_begin(nk, chunks, c + 1) * nj;
+      for (int lid = l0; lid < l1; lid++) out.line_ptr[d][lid] += base;
+      out.cross[d].insert(out.cross[d].end(), part[(size_t)c].begin(), part[(size_t)c].end());
+      std::vector<Crossing>().swap(part[(size_t)c]);
+      out.singular_warnings += bs[(size_t)c].singular_warnings;
+      out.corner_pairs[d] += bs[(size_t)c].corner_pairs;
+    }
     out.line_ptr[d][(size_t)nj * nk] = (int)out.cross[d].size();
   }
   return "";

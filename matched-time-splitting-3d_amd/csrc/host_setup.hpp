@@ -45,6 +45,9 @@ class Surface {
   virtual double dfdz(double x, double y, double z) const = 0;
   virtual double set_node(double x, double y, double z) const;
   virtual void normal(double x, double y, double z, double nrm[3]) const;
+  // Whether several threads may evaluate the surface at once (the built-in
+  // shapes; not a caller's callbacks, which run on one thread).
+  virtual bool concurrent() const { return true; }
   double dfd(int axis, double x, double y, double z) const {
     return axis == 0 ? dfdx(x, y, z) : axis == 1 ? dfdy(x, y, z) : dfdz(x, y, z);
   }
@@ -107,6 +110,7 @@ class SurfaceCallback : public Surface {
   double dfdx(double x, double y, double z) const override;
   double dfdy(double x, double y, double z) const override;
   double dfdz(double x, double y, double z) const override;
+  bool concurrent() const override { return false; }
 };
 
 // -------------------------------------------------------- diffusion beta

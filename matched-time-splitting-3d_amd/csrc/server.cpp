@@ -11,6 +11,7 @@
 #include "mts.h"
 
 #include "abi_internal.hpp"
+#include "host_parallel.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -400,8 +401,14 @@ int mts_server_create(mts_ctx *ctx, size_t hbm_cap_bytes, int max_inflight, int 
   s->ctx = ctx;
   s->cap = hbm_cap_bytes;
   s->max_inflight = max_inflight;
+  // the setup threads share the setup's thread budget: each setup splits its
+  // own work over its share
+  const int share = std::max(1, mts::setup_threads() / setup_threads);
   for (int i = 0; i < setup_threads; i++)
-    s->setup_threads.emplace_back([p = s.get()] { p->setup_loop(); });
+    s->setup_threads.emplace_back([p = s.get(), share] {
+      mts::set_setup_thread_share(share);
+      p->setup_loop();
+    });
   for (int i = 0; i < kWorkers; i++) s->workers.emplace_back([p = s.get()] { p->worker_loop(); });
   s->stepper = std::thread([p = s.get()] { p->step_loop(); });
   *out = s.release();
