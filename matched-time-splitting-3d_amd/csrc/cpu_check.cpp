@@ -39,6 +39,23 @@ struct Emu {
   PhysConst phys;
 };
 
+// Pair pe as the kernel forms it (thread pe mod kBlockThreads, step pe /
+// kBlockThreads of its TileWalk), checked against the direct mapping.
+template <int DIR, int T>
+TilePair walk_pair(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
+  const TileWalk<DIR, T> w(p, j0, k, pe % kBlockThreads);
+  const TilePair t = w.at(p, j0, k, pe / kBlockThreads, magic_nh);
+  const TilePair r = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
+  const TileAt *a[2] = {&t.e0, &t.e1}, *b[2] = {&r.e0, &r.e1};
+  for (int e = 0; e < 2; e++) {
+    if (a[e]->ok != b[e]->ok) std::abort();
+    if (a[e]->ok && (a[e]->gi != b[e]->gi || a[e]->sl != b[e]->sl || a[e]->j != b[e]->j ||
+                     a[e]->r != b[e]->r))
+      std::abort();
+  }
+  return t;
+}
+
 template <int DIR, int L, int MMAX, int FLAGS>
 void emulate_line(Emu &E, const LineArgs &p) {
   constexpr int RHS = FLAGS & 3;
@@ -67,7 +84,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
       // the kernel's paired 16-byte loads: both doubles are read whenever the
       // first element exists
       for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
-        const TilePair t = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
+        const TilePair t = walk_pair<DIR, T>(p, j0, k, pe, magic_nh);
         if (!t.e0.ok) continue;
         if (t.e0.gi % 2 != 0) std::abort();  // 16-byte alignment
         const double *fA = static_cast<const double *>(p.A);
@@ -108,7 +125,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
           }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
       for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
-        const TilePair t = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
+        const TilePair t = walk_pair<DIR, T>(p, j0, k, pe, magic_nh);
         const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
         const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
         const double v0 = t.e0.ok ? tO[t.e0.sl] : 0.0, v1 = t.e1.ok ? tO[t.e1.sl] : 0.0;

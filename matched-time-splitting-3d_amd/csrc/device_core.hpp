@@ -503,6 +503,54 @@ MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned 
   return t;
 }
 
+// The pairs of one thread, pe = tid + u * kBlockThreads for u = 0 ..
+// kTilePairsPerThread - 1, with the thread's constant part formed once per
+// tile.  y/z lines: kBlockThreads is a multiple of T / 2, so the thread
+// always moves the same two lines (2 (tid mod T/2) and the next) and its row
+// advances by kBlockThreads / (T / 2) from one u to the next.  x lines: the
+// line and row of each pair come from pe by a multiply-high, as in
+// tile_pair_at.  at(u) returns tile_pair_at(p, j0, k, tid + u * kBlockThreads).
+template <int DIR, int T>
+struct TileWalk {
+  static constexpr int kRowStep = kBlockThreads / (T / 2);  // y/z
+  size_t gb;   // y/z: global index of row 0 of the thread's first line
+  int l0;      // y/z: the thread's first line in the tile
+  int rr;      // y/z: the thread's row at u = 0
+  int tid;
+  bool okl0, okl1;  // y/z: the thread's two lines lie inside the grid
+  MTS_HD TileWalk(const LineArgs &p, int j0, int k, int tid_) : tid(tid_) {
+    using M = LineMap<DIR>;
+    if (DIR != 0) {
+      l0 = 2 * (tid & (T / 2 - 1));
+      rr = tid / (T / 2);
+      gb = M::base(p.g, j0 + l0, k);
+      okl0 = j0 + l0 < M::nj(p.g);
+      okl1 = j0 + l0 + 1 < M::nj(p.g);
+    } else {
+      l0 = rr = 0;
+      gb = 0;
+      okl0 = okl1 = false;
+    }
+  }
+  MTS_HD TilePair at(const LineArgs &p, int j0, int k, int u, unsigned magic_nh) const {
+    if (DIR == 0) return tile_pair_at<DIR, T>(p, j0, k, tid + u * kBlockThreads, magic_nh);
+    using M = LineMap<DIR>;
+    const int r = rr + u * kRowStep;
+    const bool in = r < p.g.n[DIR];
+    TilePair t;
+    t.e0.r = t.e1.r = r;
+    t.e0.j = j0 + l0;
+    t.e1.j = j0 + l0 + 1;
+    t.e0.ok = in && okl0;
+    t.e1.ok = in && okl1;
+    t.e0.gi = gb + (size_t)r * M::stride(p.g);
+    t.e1.gi = t.e0.gi + 1;
+    t.e0.sl = l0 * p.q.lds_len + lds_slot(p.q, r);
+    t.e1.sl = t.e0.sl + p.q.lds_len;
+    return t;
+  }
+};
+
 // Per-line table entries a lane needs, read once up front.
 struct LineMeta {
   int bb, be;    // block range

@@ -338,26 +338,28 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 
   // stage the tile: loads of element pairs, kChunk of them in flight
   // per thread, then their LDS stores (chunks one after the other, so the
-  // loads' registers stay bounded)
+  // loads' registers stay bounded); each pair's position is formed once
   constexpr int kChunk = 5;
   const int nh = DIR == 0 ? (n + 1) >> 1 : 1;
   const unsigned magic_nh = magic_of(nh);
+  const TileWalk<DIR, T> walk(p, j0, k, threadIdx.x);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 va[kChunk], vb[kChunk];
+    TilePair tp[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      tp[c] = walk.at(p, j0, k, u0 + c, magic_nh);
       va[c].x = va[c].y = R(0);
       vb[c].x = vb[c].y = R(0);
-      if (t.e0.ok) {
-        va[c] = *reinterpret_cast<const V2 *>(fA + t.e0.gi);
-        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + t.e0.gi);
+      if (tp[c].e0.ok) {
+        va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].e0.gi);
+        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].e0.gi);
       }
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      const TilePair &t = tp[c];
       if (t.e0.ok) {
         tA[t.e0.sl] = va[c].x;
         if (HAS_S) tB[t.e0.sl] = vb[c].x;
@@ -416,15 +418,16 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 vo[kChunk];
+    TilePair tp[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
-      vo[c].x = t.e0.ok ? tO[t.e0.sl] : R(0);
-      vo[c].y = t.e1.ok ? tO[t.e1.sl] : R(0);
+      tp[c] = walk.at(p, j0, k, u0 + c, magic_nh);
+      vo[c].x = tp[c].e0.ok ? tO[tp[c].e0.sl] : R(0);
+      vo[c].y = tp[c].e1.ok ? tO[tp[c].e1.sl] : R(0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair t = tile_pair_at<DIR, T>(p, j0, k, threadIdx.x + (u0 + c) * kBlock, magic_nh);
+      const TilePair &t = tp[c];
       const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
       const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
       if (w0 && w1) {
