@@ -39,21 +39,27 @@ struct Emu {
   PhysConst phys;
 };
 
-// Pair pe as the kernel forms it (thread pe mod kBlockThreads, step pe /
-// kBlockThreads of its TileWalk), checked against the direct mapping.
+// One pair of a thread's walk against the line and row it stands for:
+// global index, LDS slots, existence and write-back recomputed directly.
 template <int DIR, int T>
-TilePair walk_pair(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
-  const TileWalk<DIR, T> w(p, j0, k, pe % kBlockThreads);
-  const TilePair t = w.at(p, j0, k, pe / kBlockThreads, magic_nh);
-  const TilePair r = tile_pair_at<DIR, T>(p, j0, k, pe, magic_nh);
-  const TileAt *a[2] = {&t.e0, &t.e1}, *b[2] = {&r.e0, &r.e1};
-  for (int e = 0; e < 2; e++) {
-    if (a[e]->ok != b[e]->ok) std::abort();
-    if (a[e]->ok && (a[e]->gi != b[e]->gi || a[e]->sl != b[e]->sl || a[e]->j != b[e]->j ||
-                     a[e]->r != b[e]->r))
-      std::abort();
+void check_pair(const LineArgs &p, const TileWalk<DIR, T> &w, const WalkPair &t, int j0, int k) {
+  using M = LineMap<DIR>;
+  const int n = p.g.n[DIR], nj = M::nj(p.g);
+  const int j1 = DIR == 0 ? w.j : w.j + 1;      // line of the second element
+  const int r1 = DIR == 0 ? t.r + 1 : t.r;      // row of the second element
+  const bool ok0 = w.j < nj && t.r < n, ok1 = j1 < nj && r1 < n;
+  if (t.ok0 != ok0 || t.ok1 != ok1) std::abort();
+  if (t.w0 != (ok0 && w.j >= 1 && w.j <= nj - 2)) std::abort();
+  if (t.w1 != (ok1 && j1 >= 1 && j1 <= nj - 2)) std::abort();
+  if (ok0) {
+    if (t.gi != M::base(p.g, w.j, k) + (size_t)t.r * M::stride(p.g)) std::abort();
+    if (t.gi % 2 != 0) std::abort();  // 16-byte alignment of the pair
+    if (t.sl0 != (w.j - j0) * p.q.lds_len + lds_slot(p.q, t.r)) std::abort();
   }
-  return t;
+  if (ok1) {
+    if (t.gi + 1 != M::base(p.g, j1, k) + (size_t)r1 * M::stride(p.g)) std::abort();
+    if (t.sl1 != (j1 - j0) * p.q.lds_len + lds_slot(p.q, r1)) std::abort();
+  }
 }
 
 template <int DIR, int L, int MMAX, int FLAGS>
@@ -78,33 +84,40 @@ void emulate_line(Emu &E, const LineArgs &p) {
       std::memset(tB.get(), 0, sizeof(double) * T * q.lds_len);
       const int j0 = bx * T;
       if (T * n > kTileElsPerThread * kBlockThreads) std::abort();
-      const int nh = DIR == 0 ? (n + 1) >> 1 : 1;
-      const unsigned magic_nh = magic_of(nh);
       (void)magic_n;
       // the kernel's paired 16-byte loads: both doubles are read whenever the
-      // first element exists
-      for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
-        const TilePair t = walk_pair<DIR, T>(p, j0, k, pe, magic_nh);
-        if (!t.e0.ok) continue;
-        if (t.e0.gi % 2 != 0) std::abort();  // 16-byte alignment
-        const double *fA = static_cast<const double *>(p.A);
-        const double *fS = static_cast<const double *>(p.S);
-        const double a0 = fA[t.e0.gi], a1 = fA[t.e0.gi + 1];
-        double s0 = 0.0, s1 = 0.0;
-        if (HAS_S) {
-          s0 = fS[t.e0.gi];
-          s1 = fS[t.e0.gi + 1];
-        }
-        tA[t.e0.sl] = a0;
-        if (HAS_S) tB[t.e0.sl] = s0;
-        if (t.e1.ok) {
-          tA[t.e1.sl] = a1;
-          if (HAS_S) tB[t.e1.sl] = s1;
+      // first element exists; every element of the tile is moved exactly once
+      std::vector<int> seen((size_t)T * q.lds_len, 0);
+      for (int tid = 0; tid < kBlockThreads; tid++) {
+        const TileWalk<DIR, T> w(p, j0, k, tid);
+        for (int u = 0; u < kTileSteps; u++) {
+          const WalkPair t = w.at(p, u);
+          check_pair<DIR, T>(p, w, t, j0, k);
+          if (!t.ok0) continue;
+          const double *fA = static_cast<const double *>(p.A);
+          const double *fS = static_cast<const double *>(p.S);
+          const double a0 = fA[t.gi], a1 = fA[t.gi + 1];
+          double s0 = 0.0, s1 = 0.0;
+          if (HAS_S) {
+            s0 = fS[t.gi];
+            s1 = fS[t.gi + 1];
+          }
+          tA[t.sl0] = a0;
+          if (HAS_S) tB[t.sl0] = s0;
+          seen[t.sl0]++;
+          if (t.ok1) {
+            tA[t.sl1] = a1;
+            if (HAS_S) tB[t.sl1] = s1;
+            seen[t.sl1]++;
+          }
         }
       }
+      for (int l = 0; l < T && j0 + l < nj; l++)
+        for (int r = 0; r < n; r++)
+          if (seen[(size_t)l * q.lds_len + lds_slot(q, r)] != 1) std::abort();
       if (p.in_analytic)
         for (int e = 0; e < 2 * T; e++) tile_end_value<DIR>(p, j0, k, e, tA.get());
-      for (int wave = 0; wave < 4; wave++)
+      for (int wave = 0; wave < kBlockThreads / 64; wave++)
           for (int lane = 0; lane < 64; lane++) {
             const int kseg = lane & (L - 1);
             const int l = wave * LPW + lane / L;
@@ -124,13 +137,14 @@ void emulate_line(Emu &E, const LineArgs &p) {
                                            (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len);
           }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
-      for (int pe = 0; pe < kTilePairsPerThread * kBlockThreads; pe++) {
-        const TilePair t = walk_pair<DIR, T>(p, j0, k, pe, magic_nh);
-        const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
-        const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
-        const double v0 = t.e0.ok ? tO[t.e0.sl] : 0.0, v1 = t.e1.ok ? tO[t.e1.sl] : 0.0;
-        if (w0) static_cast<double *>(p.out)[t.e0.gi] = v0;
-        if (w1) static_cast<double *>(p.out)[t.e1.gi] = v1;
+      for (int tid = 0; tid < kBlockThreads; tid++) {
+        const TileWalk<DIR, T> w(p, j0, k, tid);
+        for (int u = 0; u < kTileSteps; u++) {
+          const WalkPair t = w.at(p, u);
+          const double v0 = t.w0 ? tO[t.sl0] : 0.0, v1 = t.w1 ? tO[t.sl1] : 0.0;
+          if (t.w0) static_cast<double *>(p.out)[t.gi] = v0;
+          if (t.w1) static_cast<double *>(p.out)[t.gi + 1] = v1;
+        }
       }
     }
 }

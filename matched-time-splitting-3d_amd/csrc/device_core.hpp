@@ -322,13 +322,13 @@ inline bool make_line_part(int n, LinePart &q, int min_lanes = 16) {
 }
 
 // LDS slot of row r of a line
+// (selects rather than early returns: the kernels form it per element)
 MTS_HD int lds_slot(const LinePart &q, int r) {
-  if (r <= 0) return 0;
-  if (r >= q.n - 1) return 1 + q.p * q.stride;
   const int t = r - 1;
-  int k = (t * q.magic) >> 20;
-  if (k > q.p - 1) k = q.p - 1;
-  return 1 + k * q.stride + (t - k * q.m);
+  int k = (int)(((unsigned)t * (unsigned)q.magic) >> 20);  // t / m for 0 <= t < 2^20 / m
+  k = k < q.p - 1 ? k : q.p - 1;
+  const int mid = 1 + k * q.stride + (t - k * q.m);
+  return r <= 0 ? 0 : (r >= q.n - 1 ? 1 + q.p * q.stride : mid);
 }
 
 // Segment of lane kseg of a group: first row s, m rows (0: idle lane).
@@ -468,6 +468,10 @@ struct TilePair {
 };
 // Pairs per thread that cover a tile: T * ceil(n / 2) <= 9 * 256.
 constexpr int kTilePairsPerThread = 9;
+// The kernels move them kTileChunk at a time, so steps u = 0 .. kTileSteps - 1
+// run (the last ones find no element).
+constexpr int kTileChunk = 5;
+constexpr int kTileSteps = (kTilePairsPerThread + kTileChunk - 1) / kTileChunk * kTileChunk;
 template <int DIR, int T>
 MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
   using M = LineMap<DIR>;
@@ -503,51 +507,77 @@ MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned 
   return t;
 }
 
-// The pairs of one thread, pe = tid + u * kBlockThreads for u = 0 ..
-// kTilePairsPerThread - 1, with the thread's constant part formed once per
-// tile.  y/z lines: kBlockThreads is a multiple of T / 2, so the thread
-// always moves the same two lines (2 (tid mod T/2) and the next) and its row
-// advances by kBlockThreads / (T / 2) from one u to the next.  x lines: the
-// line and row of each pair come from pe by a multiply-high, as in
-// tile_pair_at.  at(u) returns tile_pair_at(p, j0, k, tid + u * kBlockThreads).
+// The element pairs one thread moves between a tile and HBM, u = 0, 1, ...
+// (kTilePairsPerThread steps cover the tile), with everything that depends on
+// the thread only formed once per tile.  Both elements of a pair are
+// neighbours in memory (the second at global index gi + 1), so a pair is one
+// 16-byte access when both exist.
+//   y/z lines: T / 2 pairs per tile row; the thread always moves the same two
+//   neighbouring lines (2 (tid mod T/2) and the next) and its row advances by
+//   kBlockThreads / (T / 2) per step.
+//   x lines: L = kBlockThreads / T threads per line; thread tid moves rows
+//   2 (tid mod L) + 2 L u and the next of line tid / L.
+struct WalkPair {
+  size_t gi;       // global index of the first element
+  int sl0, sl1;    // LDS slots of the two elements
+  bool ok0, ok1;   // the element exists (line inside the grid, row inside the line)
+  bool w0, w1;     // and is written back (an interior line)
+  int r;           // row of the first element
+};
 template <int DIR, int T>
 struct TileWalk {
-  static constexpr int kRowStep = kBlockThreads / (T / 2);  // y/z
-  size_t gb;   // y/z: global index of row 0 of the thread's first line
-  int l0;      // y/z: the thread's first line in the tile
-  int rr;      // y/z: the thread's row at u = 0
-  int tid;
-  bool okl0, okl1;  // y/z: the thread's two lines lie inside the grid
-  MTS_HD TileWalk(const LineArgs &p, int j0, int k, int tid_) : tid(tid_) {
+  static constexpr int kL = kBlockThreads / T;  // x lines: threads per line
+  static constexpr int kRowStep = DIR == 0 ? 2 * kL : kBlockThreads / (T / 2);
+  size_t gb;          // global index of row 0 of the thread's (first) line
+  int sl;             // LDS base of that line
+  int r0;             // the thread's row at u = 0
+  int j;              // the thread's (first) line
+  bool okl0, okl1;    // its lines lie inside the grid (x: one line, twice)
+  bool wl0, wl1;      // and are interior lines
+  MTS_HD TileWalk(const LineArgs &p, int j0, int k, int tid) {
     using M = LineMap<DIR>;
-    if (DIR != 0) {
-      l0 = 2 * (tid & (T / 2 - 1));
-      rr = tid / (T / 2);
-      gb = M::base(p.g, j0 + l0, k);
-      okl0 = j0 + l0 < M::nj(p.g);
-      okl1 = j0 + l0 + 1 < M::nj(p.g);
+    const int nj = M::nj(p.g);
+    int l;
+    if (DIR == 0) {
+      l = tid / kL;
+      r0 = 2 * (tid % kL);
+      j = j0 + l;
+      okl0 = okl1 = j < nj;
+      wl0 = wl1 = j >= 1 && j <= nj - 2;
     } else {
-      l0 = rr = 0;
-      gb = 0;
-      okl0 = okl1 = false;
+      l = 2 * (tid & (T / 2 - 1));
+      r0 = tid / (T / 2);
+      j = j0 + l;
+      okl0 = j < nj;
+      okl1 = j + 1 < nj;
+      wl0 = j >= 1 && j <= nj - 2;
+      wl1 = j + 1 <= nj - 2;
     }
+    gb = M::base(p.g, j, k);
+    sl = l * p.q.lds_len;
   }
-  MTS_HD TilePair at(const LineArgs &p, int j0, int k, int u, unsigned magic_nh) const {
-    if (DIR == 0) return tile_pair_at<DIR, T>(p, j0, k, tid + u * kBlockThreads, magic_nh);
+  MTS_HD WalkPair at(const LineArgs &p, int u) const {
     using M = LineMap<DIR>;
-    const int r = rr + u * kRowStep;
-    const bool in = r < p.g.n[DIR];
-    TilePair t;
-    t.e0.r = t.e1.r = r;
-    t.e0.j = j0 + l0;
-    t.e1.j = j0 + l0 + 1;
-    t.e0.ok = in && okl0;
-    t.e1.ok = in && okl1;
-    t.e0.gi = gb + (size_t)r * M::stride(p.g);
-    t.e1.gi = t.e0.gi + 1;
-    t.e0.sl = l0 * p.q.lds_len + lds_slot(p.q, r);
-    t.e1.sl = t.e0.sl + p.q.lds_len;
-    return t;
+    const int n = p.g.n[DIR];
+    WalkPair w;
+    w.r = r0 + u * kRowStep;
+    if (DIR == 0) {
+      w.ok0 = okl0 && w.r < n;
+      w.ok1 = okl0 && w.r + 1 < n;
+      w.gi = gb + (size_t)w.r;
+      w.sl0 = sl + lds_slot(p.q, w.r);
+      w.sl1 = sl + lds_slot(p.q, w.r + 1);
+    } else {
+      const bool in = w.r < n;
+      w.ok0 = okl0 && in;
+      w.ok1 = okl1 && in;
+      w.gi = gb + (size_t)w.r * M::stride(p.g);
+      w.sl0 = sl + lds_slot(p.q, w.r);
+      w.sl1 = w.sl0 + p.q.lds_len;
+    }
+    w.w0 = w.ok0 && wl0;
+    w.w1 = w.ok1 && wl1;
+    return w;
   }
 };
 

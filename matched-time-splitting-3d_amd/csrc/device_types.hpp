@@ -131,6 +131,7 @@ enum : int {
   kFlagSolve = 8,     // tridiagonal solve along the line
   kFlagEpiF = 16,     // epilogue: x + dt f(t)  on every row (LOD last sweep)
   kFlagSeq = 32,      // some lines need the sequential Thomas solve (DevMib::seq)
+  kFlagFull = 64,     // every segment but a line's last has exactly mmax rows (set at launch)
 };
 
 struct LineArgs {

@@ -123,9 +123,15 @@ __device__ __forceinline__ float fmar(float a, float b, float c) { return fmaf(a
 // out with the next lane's first row, the one-equation-per-lane system in
 // the segment-end unknowns is solved by parallel cyclic reduction over the
 // group, and each lane back-substitutes.  On return x holds the solution.
-template <int L, int MMAX, typename R>
+// FULL: every segment but the line's last holds exactly MMAX rows.  Each lane
+// then eliminates all MMAX rows with no per-row guard: the rows past a lane's
+// m are identity rows (a = c = 0, b = 1, d = 0), decoupled from the line
+// because the last segment's last row has c = 0 (its boundary coupling is on
+// the right-hand side) and the lanes past the last segment hold nothing else.
+template <int L, int MMAX, bool FULL, typename R>
 __device__ __forceinline__ void group_solve(R (&a)[MMAX], const R (&b)[MMAX], R (&c)[MMAX],
-                                            R (&d)[MMAX], int m, int kseg) {
+                                            R (&d)[MMAX], int m_lane, int kseg) {
+  const int m = FULL ? MMAX : m_lane;
   R A1 = 0, B1 = 1, C1 = 0, D1 = 0;
   R A2 = 0, C2 = 0, D2 = 0;
   if (m >= 2) {
@@ -304,6 +310,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;  // second LDS tile: S, or the output of explicit stages
   constexpr bool SEQ = SOLVE && (FLAGS & kFlagSeq) != 0;
+  constexpr bool FULL = (FLAGS & kFlagFull) != 0;
   constexpr int LPW = 64 / L;          // lines per wavefront
   constexpr int T = tile_lines(L);     // lines per tile (all solved at once)
   const DevGeom &g = p.g;
@@ -338,35 +345,33 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 
   // stage the tile: loads of element pairs, kChunk of them in flight
   // per thread, then their LDS stores (chunks one after the other, so the
-  // loads' registers stay bounded); each pair's position is formed once
-  constexpr int kChunk = 5;
-  const int nh = DIR == 0 ? (n + 1) >> 1 : 1;
-  const unsigned magic_nh = magic_of(nh);
+  // loads' registers stay bounded)
+  constexpr int kChunk = kTileChunk;
   const TileWalk<DIR, T> walk(p, j0, k, threadIdx.x);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 va[kChunk], vb[kChunk];
-    TilePair tp[kChunk];
+    WalkPair tp[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      tp[c] = walk.at(p, j0, k, u0 + c, magic_nh);
+      tp[c] = walk.at(p, u0 + c);
       va[c].x = va[c].y = R(0);
       vb[c].x = vb[c].y = R(0);
-      if (tp[c].e0.ok) {
-        va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].e0.gi);
-        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].e0.gi);
+      if (tp[c].ok0) {
+        va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].gi);
+        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].gi);
       }
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair &t = tp[c];
-      if (t.e0.ok) {
-        tA[t.e0.sl] = va[c].x;
-        if (HAS_S) tB[t.e0.sl] = vb[c].x;
+      const WalkPair &t = tp[c];
+      if (t.ok0) {
+        tA[t.sl0] = va[c].x;
+        if (HAS_S) tB[t.sl0] = vb[c].x;
       }
-      if (t.e1.ok) {
-        tA[t.e1.sl] = va[c].y;
-        if (HAS_S) tB[t.e1.sl] = vb[c].y;
+      if (t.ok1) {
+        tA[t.sl1] = va[c].y;
+        if (HAS_S) tB[t.sl1] = vb[c].y;
       }
     }
   }
@@ -406,7 +411,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
         d[r] = hit ? (R)((double)d[r] - (double)c[r] * xn) : d[r];
         c[r] = hit ? R(0) : c[r];
       }
-      group_solve<L, MMAX, R>(a, b, c, d, sg.m, kseg);
+      group_solve<L, MMAX, FULL, R>(a, b, c, d, sg.m, kseg);
     }
     if (valid)
       line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
@@ -418,23 +423,21 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 vo[kChunk];
-    TilePair tp[kChunk];
+    WalkPair tp[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      tp[c] = walk.at(p, j0, k, u0 + c, magic_nh);
-      vo[c].x = tp[c].e0.ok ? tO[tp[c].e0.sl] : R(0);
-      vo[c].y = tp[c].e1.ok ? tO[tp[c].e1.sl] : R(0);
+      tp[c] = walk.at(p, u0 + c);
+      vo[c].x = tp[c].w0 ? tO[tp[c].sl0] : R(0);
+      vo[c].y = tp[c].w1 ? tO[tp[c].sl1] : R(0);
     }
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const TilePair &t = tp[c];
-      const bool w0 = t.e0.ok && t.e0.j >= 1 && t.e0.j <= nj - 2;
-      const bool w1 = t.e1.ok && t.e1.j >= 1 && t.e1.j <= nj - 2;
-      if (w0 && w1) {
-        *reinterpret_cast<V2 *>(fO + t.e0.gi) = vo[c];
+      const WalkPair &t = tp[c];
+      if (t.w0 && t.w1) {
+        *reinterpret_cast<V2 *>(fO + t.gi) = vo[c];
       } else {
-        if (w0) fO[t.e0.gi] = vo[c].x;
-        if (w1) fO[t.e1.gi] = vo[c].y;
+        if (t.w0) fO[t.gi] = vo[c].x;
+        if (t.w1) fO[t.gi + 1] = vo[c].y;
       }
     }
   }
@@ -456,12 +459,21 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Solve stages whose split has every segment but the last at exactly mmax
+// rows run the unguarded solve (kFlagFull; the sequential-Thomas variant keeps
+// the guarded one).
 template <int FLAGS, typename R>
 hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
   constexpr int D = MTS_LINE_DIR, LL = MTS_LINE_L;
-  if (mmax == 4) return launch_line_t<D, LL, 4, FLAGS, R>(a, st);
-  if (mmax == 8) return launch_line_t<D, LL, 8, FLAGS, R>(a, st);
-  if (mmax == 16) return launch_line_t<D, LL, 16, FLAGS, R>(a, st);
+  constexpr bool CAN_FULL = (FLAGS & kFlagSolve) && !(FLAGS & kFlagSeq);
+  constexpr int FF = CAN_FULL ? (FLAGS | kFlagFull) : FLAGS;
+  const bool full = CAN_FULL && a.q.m == mmax;
+  if (mmax == 4)
+    return full ? launch_line_t<D, LL, 4, FF, R>(a, st) : launch_line_t<D, LL, 4, FLAGS, R>(a, st);
+  if (mmax == 8)
+    return full ? launch_line_t<D, LL, 8, FF, R>(a, st) : launch_line_t<D, LL, 8, FLAGS, R>(a, st);
+  if (mmax == 16)
+    return full ? launch_line_t<D, LL, 16, FF, R>(a, st) : launch_line_t<D, LL, 16, FLAGS, R>(a, st);
   return hipErrorInvalidValue;
 }
 

@@ -238,9 +238,12 @@ static int gtsv(int n, const double *a, const double *b, const double *c, const 
  * segment is eliminated down and up keeping its first unknown as a
  * parameter, folded to one equation per lane in the segment-end unknowns,
  * solved by parallel cyclic reduction over the L lanes (idle lanes are
- * identity rows), then back-substituted. */
+ * identity rows), then back-substituted.  When every segment but the last
+ * has exactly the register bound mmax of rows, the last segment is padded to
+ * mmax rows with identity rows (zero right-hand side), as the kernel's
+ * unguarded solve eliminates them. */
 typedef struct {
-  int L, m, mlast, p;
+  int L, m, mlast, p, mmax;
 } Split;
 
 static int split_line(int n, Split *q) {
@@ -265,6 +268,7 @@ static int split_line(int n, Split *q) {
         q->m = m;
         q->mlast = mlast;
         q->p = p;
+        q->mmax = kM[mi];
         return 0;
       }
   }
@@ -282,16 +286,23 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
   if (A[0] != 0.0 || B[0] != 1.0 || C[0] != 0.0 || A[n - 1] != 0.0 || B[n - 1] != 1.0 ||
       C[n - 1] != 0.0)
     return -1;
-  double *w = (double *)malloc(sizeof(double) * (size_t)n * 3);
+  /* full split (every segment but the last has exactly mmax rows): the last
+   * segment is eliminated over m rows, its rows past the line's last interior
+   * row being identity rows with a zero right-hand side, as the kernel's
+   * unguarded solve does */
+  const int full = q.m == q.mmax;
+  const int np = n + q.m;
+  double *w = (double *)malloc(sizeof(double) * (size_t)np * 4);
   if (!w) return -1;
-  double *a = w, *c = w + n, *d = w + 2 * n;
+  double *a = w, *c = w + np, *d = w + 2 * np, *b = w + 3 * np;
   double A1[LMAX], B1[LMAX], C1[LMAX], D1[LMAX], A2[LMAX], C2[LMAX], D2[LMAX];
   double ea[LMAX], eb[LMAX], ec[LMAX], ed[LMAX], xe[LMAX];
   const int L = q.L;
-  for (int i = 0; i < n; i++) {
-    a[i] = A[i];
-    c[i] = C[i];
-    d[i] = R[i];
+  for (int i = 0; i < np; i++) {
+    a[i] = i < n ? A[i] : 0.0;
+    b[i] = i < n ? B[i] : 1.0;
+    c[i] = i < n ? C[i] : 0.0;
+    d[i] = i < n - 1 ? R[i] : 0.0;
   }
   /* boundary rows are identity rows: their values are known */
   const double x0 = R[0], xn = R[n - 1];
@@ -303,9 +314,9 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
     A1[k] = 0.0; B1[k] = 1.0; C1[k] = 0.0; D1[k] = 0.0;
     A2[k] = 0.0; C2[k] = 0.0; D2[k] = 0.0;
     if (k >= q.p) continue;
-    const int s = 1 + k * q.m, m = k == q.p - 1 ? q.mlast : q.m;
+    const int s = 1 + k * q.m, m = k == q.p - 1 && !full ? q.mlast : q.m;
     double *ra = a + s, *rc = c + s, *rd = d + s;
-    const double *rb = B + s;
+    const double *rb = b + s;
     double r = 1.0 / rb[1];
     ra[1] = ra[1] * r;
     rc[1] = rc[1] * r;
@@ -370,12 +381,13 @@ static int part_solve(int n, const double *A, const double *B, const double *C, 
   U[0] = x0;
   U[n - 1] = xn;
   for (int k = 0; k < q.p; k++) {
-    const int s = 1 + k * q.m, m = k == q.p - 1 ? q.mlast : q.m;
+    const int s = 1 + k * q.m, m = k == q.p - 1 && !full ? q.mlast : q.m;
+    const int mreal = k == q.p - 1 ? q.mlast : q.m;
     const double xp = k == 0 ? 0.0 : xe[k - 1];
     const double xs = fma(-C1[k], xe[k], fma(-A1[k], xp, D1[k])) * (1.0 / B1[k]);
     U[s] = xs;
-    for (int i = 1; i < m - 1; i++) U[s + i] = fma(-c[s + i], xe[k], fma(-a[s + i], xs, d[s + i]));
-    U[s + m - 1] = xe[k];
+    for (int i = 1; i < mreal; i++)
+      U[s + i] = i == m - 1 ? xe[k] : fma(-c[s + i], xe[k], fma(-a[s + i], xs, d[s + i]));
   }
   free(w);
   return 0;
