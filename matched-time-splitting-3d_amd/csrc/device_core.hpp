@@ -795,6 +795,58 @@ MTS_HD double block_row_rhs(const LineArgs &p, const double *w, const double (&f
   return v;
 }
 
+// Operator weights along the lines of one stage, with what is fixed along a
+// line formed once: the outside side and constant beta give constants of the
+// stage; inside a variable beta b_in (1 + 0.5 sin x cos y) has one factor fixed
+// along x and y lines (0.5 cos y, 0.5 sin x: scaling by 0.5 is exact, so the
+// products equal op_weights' bit for bit) and is constant along z lines.
+template <int DIR>
+struct LineWeights {
+  double o0, o1, o2;  // outside
+  double i0, i1, i2;  // inside (constant beta, or the z line's values)
+  double hf;          // x, y lines with variable beta: the fixed half factor
+  bool var;           // variable beta (inside weights vary along the line)
+  MTS_HD LineWeights(const PhysConst &P, const DevGeom &g, int j, int k) {
+    o0 = P.vdc[DIR][1][0];
+    o1 = P.vdc[DIR][1][1];
+    o2 = P.vdc[DIR][1][2];
+    i0 = P.vdc[DIR][0][0];
+    i1 = P.vdc[DIR][0][1];
+    i2 = P.vdc[DIR][0][2];
+    hf = 0.0;
+    var = false;
+    if (P.beta_kind != 0) {
+      if (DIR == 2) {
+        double vd[3];
+        op_weights(P, g, 2, false, j, k, 0, vd);
+        i0 = vd[0];
+        i1 = vd[1];
+        i2 = vd[2];
+      } else {
+        var = true;
+        hf = 0.5 * (DIR == 0 ? tabv(g, 1, kTabCos, j) : tabv(g, 0, kTabSin, j));
+      }
+    }
+  }
+  // weights at row i of the line on the given side (the variable inside
+  // weights are formed only where a lane needs them: a branch, which a
+  // wavefront of outside rows skips)
+  MTS_HD void at(const PhysConst &P, const DevGeom &g, bool out, int i, double vd[3]) const {
+    vd[0] = out ? o0 : i0;
+    vd[1] = out ? o1 : i1;
+    vd[2] = out ? o2 : i2;
+    if (DIR != 2 && var && !out) {
+      const int a = DIR == 0 ? 0 : 1;
+      const double tm = tabv(g, a, DIR == 0 ? kTabSinM : kTabCosM, i);
+      const double tp = tabv(g, a, DIR == 0 ? kTabSinP : kTabCosP, i);
+      const double b1 = P.b_in * (1.0 + tm * hf), b2 = P.b_in * (1.0 + tp * hf);
+      vd[0] = b1 * P.ihh[DIR];
+      vd[1] = -(b1 + b2) * P.ihh[DIR];
+      vd[2] = b2 * P.ihh[DIR];
+    }
+  }
+};
+
 // Rows [s, s + m) of line (j, k): tridiagonal coefficients and right-hand
 // side of the stage, including the line's MIB blocks (the reference's
 // per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
@@ -828,15 +880,14 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
   const LineSource<DIR> src(p.P, g, j, k);
   (void)src;
+  const LineWeights<DIR> lw(p.P, g, j, k);
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
     if (r < m) {
       const int i = s + r;
-      int ix, iy, iz;
-      M::node(j, k, i, ix, iy, iz);
       const bool out = ((cm >> r) & 1u) != 0;
       double vd[3];
-      op_weights(p.P, g, DIR, out, ix, iy, iz, vd);
+      lw.at(p.P, g, out, i, vd);
       // regular rows in the field's arithmetic (FP64, or FP32 for the FP32
       // form, whose rows are FP32 values either way)
       const R w0 = (R)vd[0], w1 = (R)vd[1], w2 = (R)vd[2];

@@ -323,7 +323,10 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     // y/z tiles narrower than a 128-byte line (T * sizeof(R) < 128): place
     // the G tiles of each line on one XCD (workgroups go to the 8 XCDs in
     // turn), dispatched one after the other, so the line is read from HBM
-    // once and the other parts come from that XCD's L2
+    // once and the other parts come from that XCD's L2.  (Otherwise
+    // neighbouring tiles stay on different XCDs at the same time: giving
+    // each XCD a contiguous range of tiles instead is slower in every
+    // direction.)
     const unsigned nb = gridDim.x * gridDim.y;
     const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
     unsigned v = b;
