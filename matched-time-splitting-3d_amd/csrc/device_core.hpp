@@ -795,51 +795,63 @@ MTS_HD double block_row_rhs(const LineArgs &p, const double *w, const double (&f
   return v;
 }
 
-// Operator weights along the lines of one stage, with what is fixed along a
-// line formed once: the outside side and constant beta give constants of the
-// stage; inside a variable beta b_in (1 + 0.5 sin x cos y) has one factor fixed
-// along x and y lines (0.5 cos y, 0.5 sin x: scaling by 0.5 is exact, so the
-// products equal op_weights' bit for bit) and is constant along z lines.
+// Operator weights along the lines of one stage.  What is fixed along a line
+// sits in a per-tile LDS table of kWtLen doubles per line, written once per
+// tile (line_weight_entry): the outside weights (entries 0..2) and the inside
+// ones (3..5: constant beta, or along z lines the variable beta's value, which
+// depends on x and y only).  A row then reads its side's three weights from
+// LDS at an address chosen by its side, rather than selecting between
+// registers.  Inside a variable beta b_in (1 + 0.5 sin x cos y) along x and y
+// lines, one factor is fixed along the line (0.5 cos y, 0.5 sin x: scaling by
+// 0.5 is exact, so the products equal op_weights' bit for bit) and the other
+// comes from the axis tables at the row.
+constexpr int kWtLen = 6;
 template <int DIR>
-struct LineWeights {
-  double o0, o1, o2;  // outside
-  double i0, i1, i2;  // inside (constant beta, or the z line's values)
-  double hf;          // x, y lines with variable beta: the fixed half factor
-  bool var;           // variable beta (inside weights vary along the line)
-  MTS_HD LineWeights(const PhysConst &P, const DevGeom &g, int j, int k) {
-    o0 = P.vdc[DIR][1][0];
-    o1 = P.vdc[DIR][1][1];
-    o2 = P.vdc[DIR][1][2];
-    i0 = P.vdc[DIR][0][0];
-    i1 = P.vdc[DIR][0][1];
-    i2 = P.vdc[DIR][0][2];
-    hf = 0.0;
-    var = false;
-    if (P.beta_kind != 0) {
-      if (DIR == 2) {
-        double vd[3];
-        op_weights(P, g, 2, false, j, k, 0, vd);
-        i0 = vd[0];
-        i1 = vd[1];
-        i2 = vd[2];
-      } else {
-        var = true;
-        hf = 0.5 * (DIR == 0 ? tabv(g, 1, kTabCos, j) : tabv(g, 0, kTabSin, j));
-      }
+MTS_HD void line_weight_entry(const LineArgs &p, int j0, int k, int e, double *wt) {
+  const int l = e / kWtLen, c = e % kWtLen;
+  const int j = j0 + l;
+  double v = 0.0;
+  if (j < LineMap<DIR>::nj(p.g)) {
+    if (c < 3) {
+      v = p.P.vdc[DIR][1][c];
+    } else if (DIR == 2 && p.P.beta_kind != 0) {
+      double vd[3];
+      op_weights(p.P, p.g, 2, false, j, k, 0, vd);
+      v = vd[c - 3];
+    } else {
+      v = p.P.vdc[DIR][0][c - 3];
     }
   }
-  // weights at row i of the line on the given side (the variable inside
+  wt[e] = v;
+}
+template <int DIR>
+struct LineWeights {
+  const double *wl;    // the line's entries of the LDS table
+  const double *tm, *tp;  // x, y lines, variable beta: axis tables at the lane's first row
+  double hf;           // and the fixed half factor
+  bool var;            // variable beta along the line (x, y lines)
+  MTS_HD LineWeights(const PhysConst &P, const DevGeom &g, const double *wl_, int j, int s)
+      : wl(wl_) {
+    tm = tp = nullptr;
+    hf = 0.0;
+    var = DIR != 2 && P.beta_kind != 0;
+    if (var) {
+      const int a = DIR == 0 ? 0 : 1;
+      hf = 0.5 * (DIR == 0 ? tabv(g, 1, kTabCos, j) : tabv(g, 0, kTabSin, j));
+      tm = g.tab[a] + (DIR == 0 ? kTabSinM : kTabCosM) * g.n[a] + s;
+      tp = g.tab[a] + (DIR == 0 ? kTabSinP : kTabCosP) * g.n[a] + s;
+    }
+  }
+  // weights at row s + r of the line on the given side (the variable inside
   // weights are formed only where a lane needs them: a branch, which a
   // wavefront of outside rows skips)
-  MTS_HD void at(const PhysConst &P, const DevGeom &g, bool out, int i, double vd[3]) const {
-    vd[0] = out ? o0 : i0;
-    vd[1] = out ? o1 : i1;
-    vd[2] = out ? o2 : i2;
+  MTS_HD void at(const PhysConst &P, bool out, int r, double vd[3]) const {
+    const double *w = wl + (out ? 0 : 3);
+    vd[0] = w[0];
+    vd[1] = w[1];
+    vd[2] = w[2];
     if (DIR != 2 && var && !out) {
-      const int a = DIR == 0 ? 0 : 1;
-      const double tm = tabv(g, a, DIR == 0 ? kTabSinM : kTabCosM, i);
-      const double tp = tabv(g, a, DIR == 0 ? kTabSinP : kTabCosP, i);
-      const double b1 = P.b_in * (1.0 + tm * hf), b2 = P.b_in * (1.0 + tp * hf);
+      const double b1 = P.b_in * (1.0 + tm[r] * hf), b2 = P.b_in * (1.0 + tp[r] * hf);
       vd[0] = b1 * P.ihh[DIR];
       vd[1] = -(b1 + b2) * P.ihh[DIR];
       vd[2] = b2 * P.ihh[DIR];
@@ -851,12 +863,13 @@ struct LineWeights {
 // side of the stage, including the line's MIB blocks (the reference's
 // per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
 // couplings of rows 1 and n-2 to the boundary rows stay in a and c.
-// la/lb: the line's input field(s) staged in LDS (slot layout of LinePart).
+// la/lb: the line's input field(s) staged in LDS (slot layout of LinePart);
+// wl: the line's entries of the tile's weight table (LineWeights).
 // R is the field type (double, or float for the FP32 form): the rows are
 // formed in FP64 and rounded to R once.
 template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const R *la,
-                      const R *lb, const LaneSeg &sg, R (&a)[MMAX],
+                      const R *lb, const double *wl, const LaneSeg &sg, R (&a)[MMAX],
                       R (&b)[MMAX], R (&c)[MMAX], R (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
@@ -880,14 +893,14 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
   const LineSource<DIR> src(p.P, g, j, k);
   (void)src;
-  const LineWeights<DIR> lw(p.P, g, j, k);
+  const LineWeights<DIR> lw(p.P, g, wl, j, s);
 #pragma unroll
   for (int r = 0; r < MMAX; r++) {
     if (r < m) {
       const int i = s + r;
       const bool out = ((cm >> r) & 1u) != 0;
       double vd[3];
-      lw.at(p.P, g, out, i, vd);
+      lw.at(p.P, out, r, vd);
       // regular rows in the field's arithmetic (FP64, or FP32 for the FP32
       // form, whose rows are FP32 values either way)
       const R w0 = (R)vd[0], w1 = (R)vd[1], w2 = (R)vd[2];

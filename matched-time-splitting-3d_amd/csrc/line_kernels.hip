@@ -378,6 +378,10 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
       }
     }
   }
+  // the tile's table of per-line operator weights (LineWeights), after the
+  // field tiles (whose size is a multiple of 8 bytes: T is even)
+  double *wt = reinterpret_cast<double *>(tA + (size_t)T * q.lds_len * (TWO ? 2 : 1));
+  if (threadIdx.x < T * kWtLen) line_weight_entry<DIR>(p, j0, k, threadIdx.x, wt);
   if (p.in_analytic) {
     // the lines' end rows take the analytic boundary value (2 per line)
     __syncthreads();
@@ -395,8 +399,8 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
     const LaneSeg sg = lane_seg(q, kseg, valid);
     R a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-    line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len, sg, a, b, c,
-                                d);
+    line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len,
+                                wt + l * kWtLen, sg, a, b, c, d);
     double x0 = 0.0, xn = 0.0;
     if (valid) line_ends<DIR>(p, j, k, x0, xn);
     if (SOLVE) {
@@ -453,7 +457,8 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   const int nk = DIR == 2 ? g.n[1] : g.n[2];
   constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
   constexpr int T = tile_lines(L);
-  const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1);
+  const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1) +
+                    (size_t)T * kWtLen * sizeof(double);
   int gx = (nj + T - 1) / T;
   constexpr int G = (int)tile_group<DIR, T, R>();
   gx = (gx + G - 1) / G * G;  // tiles in groups of G (see line_kernel)
