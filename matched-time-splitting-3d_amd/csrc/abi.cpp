@@ -383,8 +383,9 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   for (int d = 0; d < 3; d++) {
     mts::LinePart q;
     mts::make_line_part(g.n[d], q, p.min_lanes);
-    const size_t need = (size_t)mts::tile_lines(q.L) * (q.lds_len * 2 + mts::kWtLen) *
-                        sizeof(double);
+    const size_t need =
+        ((size_t)mts::tile_lines(q.L) * (q.lds_len * 2 + mts::kWtLen) + 2 * (size_t)g.n[2]) *
+        sizeof(double);
     if (need > ctx->prop.sharedMemPerBlock)
       return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
   }

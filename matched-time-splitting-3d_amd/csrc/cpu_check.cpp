@@ -117,6 +117,8 @@ void emulate_line(Emu &E, const LineArgs &p) {
           if (seen[(size_t)l * q.lds_len + lds_slot(q, r)] != 1) std::abort();
       std::unique_ptr<double[]> wt(new double[(size_t)T * kWtLen]);
       for (int e = 0; e < T * kWtLen; e++) line_weight_entry<DIR>(p, j0, k, e, wt.get());
+      std::vector<double> zt(2 * (size_t)p.g.n[2]);
+      for (int e = 0; e < 2 * p.g.n[2]; e++) ztab_entry(p.g, e, zt.data());
       if (p.in_analytic)
         for (int e = 0; e < 2 * T; e++) tile_end_value<DIR>(p, j0, k, e, tA.get());
       for (int wave = 0; wave < kBlockThreads / 64; wave++)
@@ -130,14 +132,15 @@ void emulate_line(Emu &E, const LineArgs &p) {
             if (sg.m > 0 && (sg.m < 2 || sg.m > MMAX || sg.s < 1 || sg.s + sg.m > n - 1)) std::abort();
             double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
             line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA.get() + l * q.lds_len,
-                                        tB.get() + l * q.lds_len, wt.get() + l * kWtLen, sg, a,
-                                        b, c, d);
+                                        tB.get() + l * q.lds_len, wt.get() + l * kWtLen,
+                                        DIR == 2 ? zt.data() : nullptr, sg, a, b, c, d);
             double x0 = 0.0, xn = 0.0;
             if (valid) line_ends<DIR>(p, j, k, x0, xn);
             (void)SOLVE;
             if (valid)
               line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
-                                           (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len);
+                                           (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len,
+                                           DIR == 2 ? zt.data() : nullptr);
           }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
       for (int tid = 0; tid < kBlockThreads; tid++) {

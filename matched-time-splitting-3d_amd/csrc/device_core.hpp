@@ -372,11 +372,17 @@ struct LineMap {
 // f = e sz [(3 beta - 1) sx sy - beta_x cx sy - beta_y sx cy] inside and
 // f = e (3 beta+ - 1) cx cy cz outside (the factors of source(), regrouped).
 // Other directions evaluate source() per row.
+// Along z lines sin z and cos z come from zt (sin z at rows 0..n-1, then
+// cos z), a copy of the axis tables staged in LDS, or the tables themselves
+// when zt is null.
 template <int DIR>
 struct LineSource {
   double fin = 0.0, fout = 0.0;
-  MTS_HD LineSource(const PhysConst &P, const DevGeom &g, int j, int k) {
+  const double *zs = nullptr, *zc = nullptr;
+  MTS_HD LineSource(const PhysConst &P, const DevGeom &g, int j, int k, const double *zt) {
     if (DIR == 2) {
+      zs = zt ? zt : g.tab[2] + kTabSin * g.n[2];
+      zc = zt ? zt + g.n[2] : g.tab[2] + kTabCos * g.n[2];
       const double sx = tabv(g, 0, kTabSin, j), sy = tabv(g, 1, kTabSin, k);
       const double cx = tabv(g, 0, kTabCos, j), cy = tabv(g, 1, kTabCos, k);
       fout = P.f_out * cx * cy;
@@ -392,7 +398,7 @@ struct LineSource {
   }
   MTS_HD double at(const PhysConst &P, const DevGeom &g, double e, bool out, int j, int k,
                    int i) const {
-    if (DIR == 2) return e * (out ? fout * tabv(g, 2, kTabCos, i) : fin * tabv(g, 2, kTabSin, i));
+    if (DIR == 2) return e * (out ? fout * zc[i] : fin * zs[i]);
     int ix, iy, iz;
     LineMap<DIR>::node(j, k, i, ix, iy, iz);
     return source(P, g, e, out, ix, iy, iz);
@@ -824,6 +830,12 @@ MTS_HD void line_weight_entry(const LineArgs &p, int j0, int k, int e, double *w
   }
   wt[e] = v;
 }
+// Entry e of the tile's copy of the z axis tables (LineSource): sin z at
+// rows 0..n-1, then cos z.
+MTS_HD void ztab_entry(const DevGeom &g, int e, double *zt) {
+  const int n = g.n[2];
+  zt[e] = e < n ? tabv(g, 2, kTabSin, e) : tabv(g, 2, kTabCos, e - n);
+}
 template <int DIR>
 struct LineWeights {
   const double *wl;    // the line's entries of the LDS table
@@ -864,12 +876,14 @@ struct LineWeights {
 // per-line assembly, e.g. douglas_adi.cc:377-491 and lod.cc:887-1078).  The
 // couplings of rows 1 and n-2 to the boundary rows stay in a and c.
 // la/lb: the line's input field(s) staged in LDS (slot layout of LinePart);
-// wl: the line's entries of the tile's weight table (LineWeights).
+// wl: the line's entries of the tile's weight table (LineWeights); zt: see
+// LineSource.
 // R is the field type (double, or float for the FP32 form): the rows are
 // formed in FP64 and rounded to R once.
 template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const R *la,
-                      const R *lb, const double *wl, const LaneSeg &sg, R (&a)[MMAX],
+                      const R *lb, const double *wl, const double *zt, const LaneSeg &sg,
+                      R (&a)[MMAX],
                       R (&b)[MMAX], R (&c)[MMAX], R (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
@@ -891,7 +905,7 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   // the rows just before and after the segment
   const int sl_prev = sg.k == 0 ? 0 : base - q.stride + q.m - 1;
   const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
-  const LineSource<DIR> src(p.P, g, j, k);
+  const LineSource<DIR> src(p.P, g, j, k, zt);
   (void)src;
   const LineWeights<DIR> lw(p.P, g, wl, j, s);
 #pragma unroll
@@ -965,11 +979,11 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
 // also writes the two boundary rows (x0, xn: their values).
 template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, const LaneSeg &sg,
-                       const R (&x)[MMAX], double x0, double xn, R *lo) {
+                       const R (&x)[MMAX], double x0, double xn, R *lo, const double *zt) {
   constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
   using M = LineMap<DIR>;
   const LinePart &q = p.q;
-  const LineSource<DIR> src(p.P, p.g, j, k);
+  const LineSource<DIR> src(p.P, p.g, j, k, zt);
   (void)src;
   if (sg.m > 0) {
     const unsigned cm = EPI ? rows_outside(p.mib, lm, sg.s) : 0u;

@@ -382,6 +382,13 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   // field tiles (whose size is a multiple of 8 bytes: T is even)
   double *wt = reinterpret_cast<double *>(tA + (size_t)T * q.lds_len * (TWO ? 2 : 1));
   if (threadIdx.x < T * kWtLen) line_weight_entry<DIR>(p, j0, k, threadIdx.x, wt);
+  // z lines with a source term: sin z and cos z of the rows, after the weights
+  constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || RHS == kRhsTSZ);
+  double *zt = ZT ? wt + T * kWtLen : nullptr;
+  if (ZT) {
+#pragma unroll 1
+    for (int e = threadIdx.x; e < 2 * n; e += kBlock) ztab_entry(g, e, zt);
+  }
   if (p.in_analytic) {
     // the lines' end rows take the analytic boundary value (2 per line)
     __syncthreads();
@@ -400,7 +407,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     const LaneSeg sg = lane_seg(q, kseg, valid);
     R a[MMAX], b[MMAX], c[MMAX], d[MMAX];
     line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len,
-                                wt + l * kWtLen, sg, a, b, c, d);
+                                wt + l * kWtLen, zt, sg, a, b, c, d);
     double x0 = 0.0, xn = 0.0;
     if (valid) line_ends<DIR>(p, j, k, x0, xn);
     if (SOLVE) {
@@ -422,7 +429,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     }
     if (valid)
       line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
-                                   (TWO && !HAS_S ? tB : tA) + l * q.lds_len);
+                                   (TWO && !HAS_S ? tB : tA) + l * q.lds_len, zt);
   }
   __syncthreads();
 
@@ -457,8 +464,9 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   const int nk = DIR == 2 ? g.n[1] : g.n[2];
   constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
   constexpr int T = tile_lines(L);
+  constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || (FLAGS & 3) == kRhsTSZ);
   const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1) +
-                    (size_t)T * kWtLen * sizeof(double);
+                    (size_t)T * kWtLen * sizeof(double) + (ZT ? 2 * (size_t)g.n[2] * sizeof(double) : 0);
   int gx = (nj + T - 1) / T;
   constexpr int G = (int)tile_group<DIR, T, R>();
   gx = (gx + G - 1) / G * G;  // tiles in groups of G (see line_kernel)
