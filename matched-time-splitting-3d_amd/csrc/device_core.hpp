@@ -476,7 +476,7 @@ struct TilePair {
 constexpr int kTilePairsPerThread = 9;
 // The kernels move them kTileChunk at a time, so steps u = 0 .. kTileSteps - 1
 // run (the last ones find no element).
-constexpr int kTileChunk = 5;
+constexpr int kTileChunk = 9;
 constexpr int kTileSteps = (kTilePairsPerThread + kTileChunk - 1) / kTileChunk * kTileChunk;
 template <int DIR, int T>
 MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
