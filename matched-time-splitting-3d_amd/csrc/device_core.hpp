@@ -30,6 +30,10 @@
 
 namespace mts {
 
+// a * b + c rounded once, in the arithmetic of the arguments
+MTS_HD double fmad(double a, double b, double c) { return fma(a, b, c); }
+MTS_HD float fmad(float a, float b, float c) { return fmaf(a, b, c); }
+
 // ------------------------------------------------------------ geometry
 MTS_HD double tabv(const DevGeom &g, int a, int kind, int i) {
   return g.tab[a][kind * g.n[a] + i];
@@ -929,11 +933,13 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
         const R fp = r == 0 ? f[sl_prev] : f[base + r - 1];
         const R fc = f[base + r];
         const R fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
-        const R sum = fp * w0 + fc * w1 + fn * w2;
+        // (the right-hand side with fused multiply-adds; the coefficients
+        // above round as the reference's do)
+        const R sum = fmad(fn, w2, fmad(fc, w1, fp * w0));
         if (RHS == kRhsDouglas) {
-          d[r] = -(R)p.dt * sum + la[base + r];
+          d[r] = fmad(-(R)p.dt, sum, la[base + r]);
         } else {
-          R v = rt * sum + fc;
+          R v = fmad(rt, sum, fc);
           if (RHS == kRhsTSZ) {
             v = v + (R)(p.c_f1 * src.at(p.P, g, p.e_f1, out, j, k, i));
             v = v + (R)(p.c_f2 * src.at(p.P, g, p.e_f2, out, j, k, i));

@@ -346,9 +346,8 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const R *fS = static_cast<const R *>(p.S);
   R *fO = static_cast<R *>(p.out);
 
-  // stage the tile: loads of element pairs, kChunk of them in flight
-  // per thread, then their LDS stores (chunks one after the other, so the
-  // loads' registers stay bounded)
+  // stage the tile: loads of element pairs, kChunk of them in flight per
+  // thread (all of a thread's pairs), then their LDS stores
   constexpr int kChunk = kTileChunk;
   const TileWalk<DIR, T> walk(p, j0, k, threadIdx.x);
 #pragma unroll 1
