@@ -74,7 +74,6 @@ void emulate_line(Emu &E, const LineArgs &p) {
   const int n = p.g.n[DIR];
   const int nj = LineMap<DIR>::nj(p.g), nk = LineMap<DIR>::nk(p.g);
   const int ntile = (nj + T - 1) / T;
-  const unsigned magic_n = magic_of(n);
   for (int bx = 0; bx < ntile; bx++)
     for (int k = 1; k <= nk - 2; k++) {
       // exact-size LDS tiles
@@ -84,7 +83,6 @@ void emulate_line(Emu &E, const LineArgs &p) {
       std::memset(tB.get(), 0, sizeof(double) * T * q.lds_len);
       const int j0 = bx * T;
       if (T * n > kTileElsPerThread * kBlockThreads) std::abort();
-      (void)magic_n;
       // the kernel's paired 16-byte loads: both doubles are read whenever the
       // first element exists; every element of the tile is moved exactly once
       std::vector<int> seen((size_t)T * q.lds_len, 0);

@@ -409,45 +409,6 @@ struct LineSource {
   }
 };
 
-// Tile element e -> (line l, row r) for a tile of T lines.  y/z lines: T
-// consecutive lines (one segment of an x row) per line row; x lines: one
-// contiguous line after the other (l = e / n by a multiply-high with
-// magic_n = ceil(2^32 / n)).
-template <int DIR, int T>
-MTS_HD void tile_elem(int e, int n, unsigned magic_n, int &l, int &r) {
-  if (DIR == 0) {
-    l = (int)(((unsigned long long)(unsigned)e * magic_n) >> 32);
-    r = e - l * n;
-  } else {
-    l = e & (T - 1);
-    r = e / T;
-  }
-}
-MTS_HD unsigned magic_of(int n) {
-  return (unsigned)((0x100000000ull + (unsigned long long)n - 1) / (unsigned long long)n);
-}
-
-// Tile element e: its global index, LDS slot, line and row (ok: the element
-// exists and its line is inside the grid).
-struct TileAt {
-  size_t gi;
-  int sl, j, r;
-  bool ok;
-};
-template <int DIR, int T>
-MTS_HD TileAt tile_at(const LineArgs &p, int j0, int k, int e, unsigned magic_n) {
-  using M = LineMap<DIR>;
-  const DevGeom &g = p.g;
-  const int n = g.n[DIR];
-  TileAt t;
-  int l;
-  tile_elem<DIR, T>(e, n, magic_n, l, t.r);
-  t.j = j0 + l;
-  t.ok = e < T * n && t.j < M::nj(g);
-  t.gi = M::base(g, t.j, k) + (size_t)t.r * M::stride(g);
-  t.sl = l * p.q.lds_len + lds_slot(p.q, t.r);
-  return t;
-}
 // End row e of the tile's lines (line e / 2; row 0 for even e, n - 1 for
 // odd): the analytic boundary value at the stage's input time, for stages
 // whose input end rows are boundary nodes the previous stage did not write.
@@ -466,62 +427,21 @@ MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, R *tA) {
 // make_line_part returns.
 constexpr int kTileElsPerThread = 17;
 
-// Tile elements in pairs that share one 16-byte access: two neighbouring
-// lines of one row (y/z lines; the tile's first line is even) or two
-// neighbouring rows of one line (x lines, each line taken as ceil(n/2)
-// pairs).  The second element of a pair may not exist (a line past the grid,
-// the row past an odd line's end); the 16 bytes are then still inside the
-// padded x row (the pitch is a whole number of 128-byte lines, above n when n is
-// odd), so the access stays inside the field.
-struct TilePair {
-  TileAt e0, e1;
-};
 // Pairs per thread that cover a tile: T * ceil(n / 2) <= 9 * 256.
 constexpr int kTilePairsPerThread = 9;
 // The kernels move them kTileChunk at a time, so steps u = 0 .. kTileSteps - 1
 // run (the last ones find no element).
 constexpr int kTileChunk = 9;
 constexpr int kTileSteps = (kTilePairsPerThread + kTileChunk - 1) / kTileChunk * kTileChunk;
-template <int DIR, int T>
-MTS_HD TilePair tile_pair_at(const LineArgs &p, int j0, int k, int pe, unsigned magic_nh) {
-  using M = LineMap<DIR>;
-  const DevGeom &g = p.g;
-  const int n = g.n[DIR];
-  TilePair t;
-  int l0, l1, r0, r1;
-  bool in;
-  if (DIR == 0) {
-    const int nh = (n + 1) >> 1;
-    l0 = (int)(((unsigned long long)(unsigned)pe * magic_nh) >> 32);
-    r0 = 2 * (pe - l0 * nh);
-    l1 = l0;
-    r1 = r0 + 1;
-    in = pe < T * nh;
-  } else {
-    l0 = 2 * (pe & (T / 2 - 1));
-    r0 = pe / (T / 2);
-    l1 = l0 + 1;
-    r1 = r0;
-    in = pe < (T / 2) * n;
-  }
-  t.e0.r = r0;
-  t.e0.j = j0 + l0;
-  t.e0.ok = in && t.e0.j < M::nj(g);
-  t.e0.gi = M::base(g, t.e0.j, k) + (size_t)r0 * M::stride(g);
-  t.e0.sl = l0 * p.q.lds_len + lds_slot(p.q, r0);
-  t.e1.r = r1;
-  t.e1.j = j0 + l1;
-  t.e1.ok = in && t.e1.j < M::nj(g) && r1 < n;
-  t.e1.gi = t.e0.gi + 1;
-  t.e1.sl = l1 * p.q.lds_len + lds_slot(p.q, r1);
-  return t;
-}
 
 // The element pairs one thread moves between a tile and HBM, u = 0, 1, ...
 // (kTilePairsPerThread steps cover the tile), with everything that depends on
 // the thread only formed once per tile.  Both elements of a pair are
 // neighbours in memory (the second at global index gi + 1), so a pair is one
-// 16-byte access when both exist.
+// 16-byte access whenever the first exists.  The second may not (a line past
+// the grid, the row past an odd line's end); the 16 bytes are then still
+// inside the padded x row (the pitch is a whole number of 128-byte lines,
+// above n when n is odd), so the access stays inside the field.
 //   y/z lines: T / 2 pairs per tile row; the thread always moves the same two
 //   neighbouring lines (2 (tid mod T/2) and the next) and its row advances by
 //   kBlockThreads / (T / 2) per step.
@@ -887,11 +807,9 @@ struct LineWeights {
 template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const R *la,
                       const R *lb, const double *wl, const double *zt, const LaneSeg &sg,
-                      R (&a)[MMAX],
-                      R (&b)[MMAX], R (&c)[MMAX], R (&d)[MMAX]) {
+                      R (&a)[MMAX], R (&b)[MMAX], R (&c)[MMAX], R (&d)[MMAX]) {
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
-  using M = LineMap<DIR>;
   const DevGeom &g = p.g;
   const LinePart &q = p.q;
   const double tdt = p.theta_dt;

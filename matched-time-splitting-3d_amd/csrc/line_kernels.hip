@@ -337,7 +337,6 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const int j0 = bx * T;
   const int k = by + 1;
   if (j0 >= nj) return;  // the launch rounds the grid width up to a multiple of G
-  const unsigned magic_n = magic_of(n);
 
   extern __shared__ double smem[];
   R *tA = reinterpret_cast<R *>(smem);
