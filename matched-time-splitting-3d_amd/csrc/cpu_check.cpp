@@ -51,6 +51,9 @@ void check_pair(const LineArgs &p, const TileWalk<DIR, T> &w, const WalkPair &t,
   if (t.ok0 != ok0 || t.ok1 != ok1) std::abort();
   if (t.w0 != (ok0 && w.j >= 1 && w.j <= nj - 2)) std::abort();
   if (t.w1 != (ok1 && j1 >= 1 && j1 <= nj - 2)) std::abort();
+  const bool end0 = t.r == 0 || t.r == n - 1, end1 = r1 == 0 || r1 == n - 1;
+  if (t.ld0 != (ok0 && !(p.in_analytic && end0))) std::abort();
+  if (t.ld1 != (ok1 && !(p.in_analytic && end1))) std::abort();
   if (ok0) {
     if (t.gi != M::base(p.g, w.j, k) + (size_t)t.r * M::stride(p.g)) std::abort();
     if (t.gi % 2 != 0) std::abort();  // 16-byte alignment of the pair
@@ -91,7 +94,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
         for (int u = 0; u < kTileSteps; u++) {
           const WalkPair t = w.at(p, u);
           check_pair<DIR, T>(p, w, t, j0, k);
-          if (!t.ok0) continue;
+          if (!t.ld0 && !t.ld1) continue;
           const double *fA = static_cast<const double *>(p.A);
           const double *fS = static_cast<const double *>(p.S);
           const double a0 = fA[t.gi], a1 = fA[t.gi + 1];
@@ -100,16 +103,26 @@ void emulate_line(Emu &E, const LineArgs &p) {
             s0 = fS[t.gi];
             s1 = fS[t.gi + 1];
           }
-          tA[t.sl0] = a0;
-          if (HAS_S) tB[t.sl0] = s0;
-          seen[t.sl0]++;
-          if (t.ok1) {
+          if (t.ld0) {
+            tA[t.sl0] = a0;
+            if (HAS_S) tB[t.sl0] = s0;
+            seen[t.sl0]++;
+          }
+          if (t.ld1) {
             tA[t.sl1] = a1;
             if (HAS_S) tB[t.sl1] = s1;
             seen[t.sl1]++;
           }
         }
       }
+      // in_analytic stages: the end rows come from tile_end_val, not the walk
+      if (p.in_analytic)
+        for (int e = 0; e < 2 * T; e++) {
+          const int sl = tile_end_slot<DIR>(p, j0, e);
+          if (sl < 0) continue;
+          tile_end_value<DIR>(p, j0, k, e, tA.get());
+          seen[sl]++;
+        }
       for (int l = 0; l < T && j0 + l < nj; l++)
         for (int r = 0; r < n; r++)
           if (seen[(size_t)l * q.lds_len + lds_slot(q, r)] != 1) std::abort();
@@ -117,8 +130,6 @@ void emulate_line(Emu &E, const LineArgs &p) {
       for (int e = 0; e < T * kWtLen; e++) line_weight_entry<DIR>(p, j0, k, e, wt.get());
       std::vector<double> zt(2 * (size_t)p.g.n[2]);
       for (int e = 0; e < 2 * p.g.n[2]; e++) ztab_entry(p.g, e, zt.data());
-      if (p.in_analytic)
-        for (int e = 0; e < 2 * T; e++) tile_end_value<DIR>(p, j0, k, e, tA.get());
       for (int wave = 0; wave < kBlockThreads / 64; wave++)
           for (int lane = 0; lane < 64; lane++) {
             const int kseg = lane & (L - 1);

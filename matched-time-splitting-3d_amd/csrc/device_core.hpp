@@ -412,16 +412,30 @@ struct LineSource {
 // End row e of the tile's lines (line e / 2; row 0 for even e, n - 1 for
 // odd): the analytic boundary value at the stage's input time, for stages
 // whose input end rows are boundary nodes the previous stage did not write.
-template <int DIR, typename R>
-MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, R *tA) {
+// The tile walk skips these rows when in_analytic is set (their loaded values
+// would be replaced), so the kernel computes the values before the tile
+// loads and stores them with the tile, with no ordering between the two.
+// tile_end_slot: the LDS slot of end row e, or -1 if its line lies past the grid.
+template <int DIR>
+MTS_HD int tile_end_slot(const LineArgs &p, int j0, int e) {
   const int n = p.g.n[DIR];
   const int l = e >> 1;
-  const int j = j0 + l;
-  if (j >= LineMap<DIR>::nj(p.g)) return;
-  const int r = (e & 1) ? n - 1 : 0;
+  if (j0 + l >= LineMap<DIR>::nj(p.g)) return -1;
+  return l * p.q.lds_len + lds_slot(p.q, (e & 1) ? n - 1 : 0);
+}
+template <int DIR>
+MTS_HD double tile_end_val(const LineArgs &p, int j0, int k, int e) {
+  const int n = p.g.n[DIR];
+  const int j = j0 + (e >> 1);
+  if (j >= LineMap<DIR>::nj(p.g)) return 0.0;
   int ix, iy, iz;
-  LineMap<DIR>::node(j, k, r, ix, iy, iz);
-  tA[l * p.q.lds_len + lds_slot(p.q, r)] = (R)outer_u(p.g, p.e_in, ix, iy, iz);
+  LineMap<DIR>::node(j, k, (e & 1) ? n - 1 : 0, ix, iy, iz);
+  return outer_u(p.g, p.e_in, ix, iy, iz);
+}
+template <int DIR, typename R>
+MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, R *tA) {
+  const int sl = tile_end_slot<DIR>(p, j0, e);
+  if (sl >= 0) tA[sl] = (R)tile_end_val<DIR>(p, j0, k, e);
 }
 // Elements per thread that cover a tile: T * n <= 17 * 256 for every split
 // make_line_part returns.
@@ -452,6 +466,7 @@ struct WalkPair {
   int sl0, sl1;    // LDS slots of the two elements
   bool ok0, ok1;   // the element exists (line inside the grid, row inside the line)
   bool w0, w1;     // and is written back (an interior line)
+  bool ld0, ld1;   // and is loaded (not an input end row of an in_analytic stage)
   int r;           // row of the first element
 };
 template <int DIR, int T>
@@ -505,8 +520,18 @@ struct TileWalk {
       w.sl0 = sl + lds_slot(p.q, w.r);
       w.sl1 = w.sl0 + p.q.lds_len;
     }
+    // the write-back covers the end rows either way (the stage's output)
     w.w0 = w.ok0 && wl0;
     w.w1 = w.ok1 && wl1;
+    if (p.in_analytic) {
+      // input end rows come from tile_end_val instead
+      const int r1 = DIR == 0 ? w.r + 1 : w.r;
+      w.ld0 = w.ok0 && w.r != 0 && w.r != n - 1;
+      w.ld1 = w.ok1 && r1 != 0 && r1 != n - 1;
+    } else {
+      w.ld0 = w.ok0;
+      w.ld1 = w.ok1;
+    }
     return w;
   }
 };
@@ -737,7 +762,7 @@ MTS_HD double block_row_rhs(const LineArgs &p, const double *w, const double (&f
 // comes from the axis tables at the row.
 constexpr int kWtLen = 6;
 template <int DIR>
-MTS_HD void line_weight_entry(const LineArgs &p, int j0, int k, int e, double *wt) {
+MTS_HD double line_weight_value(const LineArgs &p, int j0, int k, int e) {
   const int l = e / kWtLen, c = e % kWtLen;
   const int j = j0 + l;
   double v = 0.0;
@@ -752,14 +777,19 @@ MTS_HD void line_weight_entry(const LineArgs &p, int j0, int k, int e, double *w
       v = p.P.vdc[DIR][0][c - 3];
     }
   }
-  wt[e] = v;
+  return v;
+}
+template <int DIR>
+MTS_HD void line_weight_entry(const LineArgs &p, int j0, int k, int e, double *wt) {
+  wt[e] = line_weight_value<DIR>(p, j0, k, e);
 }
 // Entry e of the tile's copy of the z axis tables (LineSource): sin z at
 // rows 0..n-1, then cos z.
-MTS_HD void ztab_entry(const DevGeom &g, int e, double *zt) {
+MTS_HD double ztab_value(const DevGeom &g, int e) {
   const int n = g.n[2];
-  zt[e] = e < n ? tabv(g, 2, kTabSin, e) : tabv(g, 2, kTabCos, e - n);
+  return e < n ? tabv(g, 2, kTabSin, e) : tabv(g, 2, kTabCos, e - n);
 }
+MTS_HD void ztab_entry(const DevGeom &g, int e, double *zt) { zt[e] = ztab_value(g, e); }
 template <int DIR>
 struct LineWeights {
   const double *wl;    // the line's entries of the LDS table

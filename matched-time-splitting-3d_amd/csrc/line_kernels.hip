@@ -348,7 +348,27 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   // stage the tile: loads of element pairs, kChunk of them in flight per
   // thread (all of a thread's pairs), then their LDS stores
   constexpr int kChunk = kTileChunk;
-  const TileWalk<DIR, T> walk(p, j0, k, threadIdx.x);
+  const int tid = threadIdx.x;
+  // The small tables that join the tile (LDS after the field tiles, whose
+  // size is a multiple of 8 bytes: T is even): the per-line operator weights
+  // (LineWeights); for z lines with a source term, sin z and cos z of the
+  // rows; for in_analytic stages, the lines' end rows (2 per line), which the
+  // tile walk does not load.  Their global loads go out before the tile's,
+  // so the two latencies overlap.
+  constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || RHS == kRhsTSZ);
+  constexpr int kZPre = 3;  // z-table entries a thread loads ahead of the tile
+  const double wt_v = tid < T * kWtLen ? line_weight_value<DIR>(p, j0, k, tid) : 0.0;
+  double zt_v[kZPre];
+#pragma unroll
+  for (int i = 0; i < kZPre; i++) {
+    const int e = tid + i * kBlock;
+    zt_v[i] = ZT && e < 2 * n ? ztab_value(g, e) : 0.0;
+  }
+  const bool end_row = p.in_analytic && tid < 2 * T;
+  const int end_sl = end_row ? tile_end_slot<DIR>(p, j0, tid) : -1;
+  const double end_v = end_sl >= 0 ? tile_end_val<DIR>(p, j0, k, tid) : 0.0;
+
+  const TileWalk<DIR, T> walk(p, j0, k, tid);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 va[kChunk], vb[kChunk];
@@ -358,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
       tp[c] = walk.at(p, u0 + c);
       va[c].x = va[c].y = R(0);
       vb[c].x = vb[c].y = R(0);
-      if (tp[c].ok0) {
+      if (tp[c].ld0 || tp[c].ld1) {  // (ld1 implies the first element exists)
         va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].gi);
         if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].gi);
       }
@@ -366,32 +386,29 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
       const WalkPair &t = tp[c];
-      if (t.ok0) {
+      if (t.ld0) {
         tA[t.sl0] = va[c].x;
         if (HAS_S) tB[t.sl0] = vb[c].x;
       }
-      if (t.ok1) {
+      if (t.ld1) {
         tA[t.sl1] = va[c].y;
         if (HAS_S) tB[t.sl1] = vb[c].y;
       }
     }
   }
-  // the tile's table of per-line operator weights (LineWeights), after the
-  // field tiles (whose size is a multiple of 8 bytes: T is even)
   double *wt = reinterpret_cast<double *>(tA + (size_t)T * q.lds_len * (TWO ? 2 : 1));
-  if (threadIdx.x < T * kWtLen) line_weight_entry<DIR>(p, j0, k, threadIdx.x, wt);
-  // z lines with a source term: sin z and cos z of the rows, after the weights
-  constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || RHS == kRhsTSZ);
   double *zt = ZT ? wt + T * kWtLen : nullptr;
+  if (tid < T * kWtLen) wt[tid] = wt_v;
   if (ZT) {
+#pragma unroll
+    for (int i = 0; i < kZPre; i++) {
+      const int e = tid + i * kBlock;
+      if (e < 2 * n) zt[e] = zt_v[i];
+    }
 #pragma unroll 1
-    for (int e = threadIdx.x; e < 2 * n; e += kBlock) ztab_entry(g, e, zt);
+    for (int e = tid + kZPre * kBlock; e < 2 * n; e += kBlock) ztab_entry(g, e, zt);
   }
-  if (p.in_analytic) {
-    // the lines' end rows take the analytic boundary value (2 per line)
-    __syncthreads();
-    if (threadIdx.x < 2 * T) tile_end_value<DIR>(p, j0, k, threadIdx.x, tA);
-  }
+  if (end_sl >= 0) tA[end_sl] = (R)end_v;
   __syncthreads();
 
   const int lane = threadIdx.x & 63;
