@@ -364,6 +364,18 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
     const int e = tid + i * kBlock;
     zt_v[i] = ZT && e < 2 * n ? ztab_value(g, e) : 0.0;
   }
+  // this lane's line: its table entries and boundary values, also loaded
+  // ahead of the tile
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kseg = lane & (L - 1);
+  const int l = wave * LPW + lane / L;
+  const int j = j0 + l;
+  const bool valid = j >= 1 && j <= nj - 2;
+  const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
+  const LaneSeg sg = lane_seg(q, kseg, valid);
+  double x0 = 0.0, xn = 0.0;
+  if (valid) line_ends<DIR>(p, j, k, x0, xn);
   const bool end_row = p.in_analytic && tid < 2 * T;
   const int end_sl = end_row ? tile_end_slot<DIR>(p, j0, tid) : -1;
   const double end_v = end_sl >= 0 ? tile_end_val<DIR>(p, j0, k, tid) : 0.0;
@@ -411,20 +423,10 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   if (end_sl >= 0) tA[end_sl] = (R)end_v;
   __syncthreads();
 
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int kseg = lane & (L - 1);
   {
-    const int l = wave * LPW + lane / L;
-    const int j = j0 + l;
-    const bool valid = j >= 1 && j <= nj - 2;
-    const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
-    const LaneSeg sg = lane_seg(q, kseg, valid);
     R a[MMAX], b[MMAX], c[MMAX], d[MMAX];
     line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA + l * q.lds_len, tB + l * q.lds_len,
                                 wt + l * kWtLen, zt, sg, a, b, c, d);
-    double x0 = 0.0, xn = 0.0;
-    if (valid) line_ends<DIR>(p, j, k, x0, xn);
     if (SOLVE) {
       if (SEQ) {
         if (__any(lm.seq)) group_thomas<L, MMAX, R>(a, b, c, d, q, lane, kseg, lm.seq, x0, xn);
