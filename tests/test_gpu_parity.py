@@ -1,6 +1,6 @@
 """GPU parity: the HIP path against the reference's golden fields and the oracle.
 
-Tolerance: 1e-12 normwise relative (north-star bar for the plain FP64 form),
+Tolerance: 1e-12 normwise relative (the bar for the plain FP64 form),
 scaled by the solution's growth for runs the reference's scheme amplifies
 (TS and Douglas at dt = h with strong contrast grow by up to 1e7 over a few
 steps and amplify every rounding difference by as much).  The partitioned

@@ -62,7 +62,7 @@ def test_indicator_flips_at_crossings(mts, name):
 
 
 def test_corner_pairs_present(mts):
-    """The configs[0] geometry produces corner pairs in MIB-L1 (exercised by parity)."""
+    """The golden geometries include corner pairs (exercised by parity)."""
     total = 0
     for name in GOLDEN_NAMES:
         g = load_golden(name)
