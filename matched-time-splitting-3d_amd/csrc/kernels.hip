@@ -171,25 +171,27 @@ __global__ __launch_bounds__(256) void spec_kernel(SpecArgs p) {
 template <typename R>
 __global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *out, double e,
                                                     double cf) {
+  // (32-bit index arithmetic: the six faces hold at most 6 * 1024^2 nodes,
+  // and 64-bit division is a long software sequence on the GPU)
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  const long long fz = (long long)nx * ny, fy = (long long)nx * nz, fx = (long long)ny * nz;
-  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int fz = nx * ny, fy = nx * nz, fx = ny * nz;
+  int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   int ix, iy, iz;
   if (t < 2 * fz) {
     iz = t < fz ? 0 : nz - 1;
-    long long q = t % fz;
-    ix = (int)(q % nx);
-    iy = (int)(q / nx);
+    const int q = t < fz ? t : t - fz;
+    iy = q / nx;
+    ix = q - iy * nx;
   } else if ((t -= 2 * fz) < 2 * fy) {
     iy = t < fy ? 0 : ny - 1;
-    long long q = t % fy;
-    ix = (int)(q % nx);
-    iz = (int)(q / nx);
+    const int q = t < fy ? t : t - fy;
+    iz = q / nx;
+    ix = q - iz * nx;
   } else if ((t -= 2 * fy) < 2 * fx) {
     ix = t < fx ? 0 : nx - 1;
-    long long q = t % fx;
-    iy = (int)(q % ny);
-    iz = (int)(q / ny);
+    const int q = t < fx ? t : t - fx;
+    iz = q / ny;
+    iy = q - iz * ny;
   } else {
     return;
   }
