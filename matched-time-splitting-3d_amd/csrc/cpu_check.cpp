@@ -1,16 +1,23 @@
-// CPU bounds check of the per-step kernels' memory accesses.
+// CPU check of the per-step kernels: memory accesses and values.
 //
 // Builds the device table for one geometry exactly as an instance does, then
-// runs every global-memory and LDS access of one time step of the chosen
-// method on the host, using the same per-node / per-lane functions the HIP
-// kernels call (device_core.hpp), over buffers of exactly the device sizes.
-// Compiled with AddressSanitizer, so any index a kernel would form outside
-// its buffer aborts here instead of faulting a GPU.  The tridiagonal solve
-// itself touches registers only and is not emulated (rows keep their RHS).
+// runs one time step of the chosen method on the host, using the same
+// per-node / per-lane functions the HIP kernels call (device_core.hpp), over
+// buffers of exactly the device sizes.  Compiled with AddressSanitizer, so
+// any index a kernel would form outside its buffer aborts here instead of
+// faulting a GPU.  Every line stage is emulated end to end: the tile walk,
+// the rows each lane builds (line_rows), the boundary couplings moved to the
+// right-hand side as the kernel does, a sequential Thomas solve of the line
+// on the host in place of the lane-parallel one (which touches registers
+// only), and the write-back through line_write.  With --dump DIR the step
+// starts from the exact initial condition at t = 0 with the time factors of
+// the first step, and the fields after each stage are written to DIR for a
+// comparison with the oracle.
 //
 //   mts_cpucheck <method adi|lod-ie|lod-cn|ts> <mib 1|2> <nodes per axis>
 //                <surface ellipsoid|torus|bumpy> <p0> <p1> [p2 p3]
-//                <beta const|sincos> <b0> <b1>
+//                <beta const|sincos> <b0> <b1> [--dump DIR]
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -71,7 +78,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;
-  constexpr int LPW = 64 / L, T = tile_lines(L);
+  constexpr int T = tile_lines(L);
   (void)E;
   const LinePart &q = p.q;
   const int n = p.g.n[DIR];
@@ -130,27 +137,70 @@ void emulate_line(Emu &E, const LineArgs &p) {
       for (int e = 0; e < T * kWtLen; e++) line_weight_entry<DIR>(p, j0, k, e, wt.get());
       std::vector<double> zt(2 * (size_t)p.g.n[2]);
       for (int e = 0; e < 2 * p.g.n[2]; e++) ztab_entry(p.g, e, zt.data());
-      for (int wave = 0; wave < kBlockThreads / 64; wave++)
-          for (int lane = 0; lane < 64; lane++) {
-            const int kseg = lane & (L - 1);
-            const int l = wave * LPW + lane / L;
-            const int j = j0 + l;
-            const bool valid = j >= 1 && j <= nj - 2;
-            const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
-            const LaneSeg sg = lane_seg(q, kseg, valid);
-            if (sg.m > 0 && (sg.m < 2 || sg.m > MMAX || sg.s < 1 || sg.s + sg.m > n - 1)) std::abort();
-            double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
-            line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA.get() + l * q.lds_len,
-                                        tB.get() + l * q.lds_len, wt.get() + l * kWtLen,
-                                        DIR == 2 ? zt.data() : nullptr, sg, a, b, c, d);
-            double x0 = 0.0, xn = 0.0;
-            if (valid) line_ends<DIR>(p, j, k, x0, xn);
-            (void)SOLVE;
-            if (valid)
-              line_write<DIR, MMAX, FLAGS>(p, j, k, lm, sg, d, x0, xn,
-                                           (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len,
-                                           DIR == 2 ? zt.data() : nullptr);
+      for (int l = 0; l < T; l++) {
+        const int j = j0 + l;
+        const bool valid = j >= 1 && j <= nj - 2;
+        const LineMeta lm = line_meta(p.mib, k * nj + j, valid);
+        // the rows of every lane of the line's group, then the line solved as
+        // the kernel solves it (boundary couplings on the right-hand side)
+        std::vector<double> la(L * MMAX), lb(L * MMAX), lc(L * MMAX), ld(L * MMAX);
+        std::vector<LaneSeg> segs(L);
+        for (int kseg = 0; kseg < L; kseg++) {
+          const LaneSeg sg = lane_seg(q, kseg, valid);
+          segs[kseg] = sg;
+          if (sg.m > 0 && (sg.m < 2 || sg.m > MMAX || sg.s < 1 || sg.s + sg.m > n - 1)) std::abort();
+          double a[MMAX], b[MMAX], c[MMAX], d[MMAX];
+          line_rows<DIR, MMAX, FLAGS>(p, j, k, lm, tA.get() + l * q.lds_len,
+                                      tB.get() + l * q.lds_len, wt.get() + l * kWtLen,
+                                      DIR == 2 ? zt.data() : nullptr, sg, a, b, c, d);
+          for (int r = 0; r < MMAX; r++) {
+            la[kseg * MMAX + r] = a[r];
+            lb[kseg * MMAX + r] = b[r];
+            lc[kseg * MMAX + r] = c[r];
+            ld[kseg * MMAX + r] = d[r];
           }
+        }
+        if (!valid) continue;
+        double x0 = 0.0, xn = 0.0;
+        line_ends<DIR>(p, j, k, x0, xn);
+        if (SOLVE) {
+          // interior rows 1 .. n-2 in line order
+          std::vector<double> A(n), B(n), C(n), D(n), X(n);
+          std::vector<int> seen_row(n, 0);
+          for (int kseg = 0; kseg < L; kseg++)
+            for (int r = 0; r < segs[kseg].m; r++) {
+              const int i = segs[kseg].s + r;
+              A[i] = la[kseg * MMAX + r];
+              B[i] = lb[kseg * MMAX + r];
+              C[i] = lc[kseg * MMAX + r];
+              D[i] = ld[kseg * MMAX + r];
+              seen_row[i]++;
+            }
+          for (int i = 1; i <= n - 2; i++)
+            if (seen_row[i] != 1) std::abort();  // the segments cover the interior exactly once
+          D[1] -= A[1] * x0;
+          D[n - 2] -= C[n - 2] * xn;
+          // Thomas over rows 1 .. n-2 (the reference's TDMA order)
+          std::vector<double> gam(n);
+          double bet = B[1];
+          X[1] = D[1] / bet;
+          for (int i = 2; i <= n - 2; i++) {
+            gam[i] = C[i - 1] / bet;
+            bet = B[i] - A[i] * gam[i];
+            X[i] = (D[i] - A[i] * X[i - 1]) / bet;
+          }
+          for (int i = n - 3; i >= 1; i--) X[i] -= gam[i + 1] * X[i + 1];
+          for (int kseg = 0; kseg < L; kseg++)
+            for (int r = 0; r < segs[kseg].m; r++) ld[kseg * MMAX + r] = X[segs[kseg].s + r];
+        }
+        for (int kseg = 0; kseg < L; kseg++) {
+          double x[MMAX];
+          for (int r = 0; r < MMAX; r++) x[r] = ld[kseg * MMAX + r];
+          line_write<DIR, MMAX, FLAGS>(p, j, k, lm, segs[kseg], x, x0, xn,
+                                       (TWO && !HAS_S ? tB.get() : tA.get()) + l * q.lds_len,
+                                       DIR == 2 ? zt.data() : nullptr);
+        }
+      }
       const double *tO = TWO && !HAS_S ? tB.get() : tA.get();
       for (int tid = 0; tid < kBlockThreads; tid++) {
         const TileWalk<DIR, T> w(p, j0, k, tid);
@@ -198,11 +248,39 @@ void line(Emu &E, int dir, const LineArgs &args) {
   else std::abort();
 }
 
+// Field (x-pitched) to nx*ny*nz, x fastest, written as raw float64.
+void dump_field(const std::string &path, const std::vector<double> &f, const int nn[3], int px) {
+  std::vector<double> out((size_t)nn[0] * nn[1] * nn[2]);
+  for (int iz = 0; iz < nn[2]; iz++)
+    for (int iy = 0; iy < nn[1]; iy++)
+      std::memcpy(&out[((size_t)iz * nn[1] + iy) * nn[0]], &f[((size_t)iz * nn[1] + iy) * px],
+                  nn[0] * sizeof(double));
+  FILE *fp = std::fopen(path.c_str(), "wb");
+  if (!fp || std::fwrite(out.data(), sizeof(double), out.size(), fp) != out.size()) std::abort();
+  std::fclose(fp);
+}
+
+// Boundary faces of one step (faces_kernel).
+void faces(const Emu &E, std::vector<double> &u, double e, double cf) {
+  for (int iz = 0; iz < E.g.n[2]; iz++)
+    for (int iy = 0; iy < E.g.n[1]; iy++)
+      for (int ix = 0; ix < E.g.n[0]; ix++) {
+        const bool face = ix == 0 || iy == 0 || iz == 0 || ix == E.g.n[0] - 1 ||
+                          iy == E.g.n[1] - 1 || iz == E.g.n[2] - 1;
+        if (face) u[node_index(E.g, ix, iy, iz)] = face_value(E.phys, E.g, e, cf, ix, iy, iz);
+      }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
+  std::string dump_dir;
+  if (argc > 2 && std::string(argv[argc - 2]) == "--dump") {
+    dump_dir = argv[argc - 1];
+    argc -= 2;
+  }
   if (argc < 9) {
-    std::fprintf(stderr, "usage: mts_cpucheck method mib nodes surface p.. beta b0 b1\n");
+    std::fprintf(stderr, "usage: mts_cpucheck method mib nodes surface p.. beta b0 b1 [--dump DIR]\n");
     return 2;
   }
   std::string method = argv[1];
@@ -263,13 +341,30 @@ int main(int argc, char **argv) {
   }
   E.g.px = E.px;
   E.g.ind = E.ind.data();
+  E.phys = E.t.phys;
   for (int d = 0; d < 3; d++) {
+    // a direction without blocks holds no table (the device buffers of size
+    // zero are null pointers): the kernels must not read it
+    const bool any = E.t.n_blocks[d] > 0;
     E.mib[d].blk_ptr = E.t.blk_ptr[d].data();
-    E.mib[d].hdr = E.t.blk_hdr[d].data();
-    E.mib[d].val = E.t.blk_val[d].data();
-    E.mib[d].cr = E.t.cr_val[d].data();
+    E.mib[d].hdr = any ? E.t.blk_hdr[d].data() : nullptr;
+    E.mib[d].val = any ? E.t.blk_val[d].data() : nullptr;
+    E.mib[d].cr = E.t.cr_val[d].empty() ? nullptr : E.t.cr_val[d].data();
     E.mib[d].seq = E.t.n_seq[d] > 0 ? E.t.seq[d].data() : nullptr;
     E.mib[d].cls0 = E.t.cls0[d].data();
+  }
+  // time factors: the first step from t = 0 (mts_step with t_next = dt)
+  const double t1 = dt;
+  const double e1 = std::exp(-t1), en = std::exp(-(t1 - dt)), eh = std::exp(-(t1 - 0.5 * dt));
+  const bool dump = !dump_dir.empty();
+  if (dump) {
+    for (int iz = 0; iz < nn[2]; iz++)
+      for (int iy = 0; iy < nn[1]; iy++)
+        for (int ix = 0; ix < nn[0]; ix++) {
+          const size_t c = node_index(E.g, ix, iy, iz);
+          E.u[c] = E.ind[c] > 0 ? outer_u(E.g, 1.0, ix, iy, iz) : inner_u(E.g, 1.0, ix, iy, iz);
+        }
+    dump_field(dump_dir + "/u0.bin", E.u, nn, E.px);
   }
   LineArgs a;
   std::memset(&a, 0, sizeof(a));
@@ -279,10 +374,8 @@ int main(int argc, char **argv) {
   a.P = E.t.phys;
   a.theta_dt = tdt;
   a.dt = dt;
-  a.e_out = a.e_in = a.e_jump = a.e_f1 = a.e_f2 = 0.9;
-  a.c_f1 = a.c_f2 = 0.5 * dt;
   if (adi) {
-    Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, 0.9};
+    Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, e1};
     for (int iz = 0; iz < nn[2]; iz++)
       for (int iy = 0; iy < nn[1]; iy++)
         for (int ix = 0; ix < nn[0]; ix++) rhs1_node(r1, ix, iy, iz);
@@ -295,24 +388,39 @@ int main(int argc, char **argv) {
     sp.u = E.u.data();
     sp.R = E.r.data();
     sp.dt = dt;
-    sp.e = 0.9;
+    sp.e = e1;
     for (int t = 0; t < sp.nspec; t++) spec_node(sp, t);
+    if (dump) dump_field(dump_dir + "/rhs1.bin", E.r, nn, E.px);
+    a.e_out = e1;
+    a.e_jump = e1;
     a.A = E.r.data();
     a.out = E.r.data();
     a.mib = E.mib[0];
     line<kRhsCopy | kFlagSolve>(E, 0, a);
+    if (dump) dump_field(dump_dir + "/ustar.bin", E.r, nn, E.px);
     a.S = E.u.data();
     a.mib = E.mib[1];
     line<kRhsDouglas | kFlagSolve>(E, 1, a);
+    if (dump) dump_field(dump_dir + "/ustar2.bin", E.r, nn, E.px);
     a.mib = E.mib[2];
     a.out = E.u.data();
     line<kRhsDouglas | kFlagSolve>(E, 2, a);
+    faces(E, E.u, e1, 0.0);
   } else if (method == "lod-ie" || method == "lod-cn") {
+    a.e_out = e1;
+    a.e_jump = e1;
     a.A = a.out = E.u.data();
     const bool cn = method == "lod-cn";
     for (int d = 0; d < 3; d++) {
       a.mib = E.mib[d];
-      a.in_analytic = d > 0;
+      if (d == 1) {
+        a.in_analytic = 1;
+        a.e_in = e1;
+      }
+      if (d == 2) {
+        a.e_f1 = e1;
+        a.c_f1 = dt;
+      }
       if (cn) {
         if (d < 2) line<kRhsCN | kFlagSolve | kFlagLhsJump>(E, d, a);
         else line<kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF>(E, d, a);
@@ -321,22 +429,34 @@ int main(int argc, char **argv) {
         else line<kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF>(E, d, a);
       }
     }
+    faces(E, E.u, e1, dt);
   } else if (method == "ts") {
+    a.e_jump = e1;
     a.A = a.out = E.u.data();
+    a.e_out = eh;
     a.mib = E.mib[0];
     line<kRhsCN>(E, 0, a);
     a.in_analytic = 1;
+    a.e_in = eh;
     a.mib = E.mib[1];
     line<kRhsCN>(E, 1, a);
     a.mib = E.mib[2];
+    a.e_out = e1;
+    a.e_f1 = en;
+    a.c_f1 = 0.5 * dt;
+    a.e_f2 = e1;
+    a.c_f2 = 0.5 * dt;
     line<kRhsTSZ | kFlagSolve | kFlagLhsJump>(E, 2, a);
+    a.e_in = e1;
     a.mib = E.mib[1];
     line<kRhsCopy | kFlagSolve | kFlagLhsJump>(E, 1, a);
     a.mib = E.mib[0];
     line<kRhsCopy | kFlagSolve | kFlagLhsJump>(E, 0, a);
+    faces(E, E.u, e1, 0.0);
   } else {
     return 2;
   }
+  if (dump) dump_field(dump_dir + "/u1.bin", E.u, nn, E.px);
   std::printf("ok %s mib%d %dx%dx%d crossings %zu %zu %zu\n", method.c_str(), mib, nn[0], nn[1], nn[2],
               res.cross[0].size(), res.cross[1].size(), res.cross[2].size());
   return 0;

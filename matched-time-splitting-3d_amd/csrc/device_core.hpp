@@ -68,6 +68,15 @@ MTS_HD double source(const PhysConst &P, const DevGeom &g, double e,
   return (3.0 * b - 1.0) * u - bx * ux - by * uy;
 }
 
+// Value of boundary node (ix, iy, iz): the exact outer solution at e, plus
+// cf f(e) when cf != 0 (LOD's source epilogue reaches the boundary too).
+MTS_HD double face_value(const PhysConst &P, const DevGeom &g, double e, double cf, int ix,
+                         int iy, int iz) {
+  double v = outer_u(g, e, ix, iy, iz);
+  if (cf != 0.0) v = v + cf * source(P, g, e, g.ind[node_index(g, ix, iy, iz)] > 0, ix, iy, iz);
+  return v;
+}
+
 // Operator weights beta(x -+ h/2)/h^2 along axis d on the node's side.
 MTS_HD void op_weights(const PhysConst &P, const DevGeom &g, int d,
                                            bool outside, int ix, int iy, int iz, double vd[3]) {

@@ -195,10 +195,7 @@ __global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *o
   } else {
     return;
   }
-  const size_t c = node_index(g, ix, iy, iz);
-  double v = outer_u(g, e, ix, iy, iz);
-  if (cf != 0.0) v = v + cf * source(P, g, e, g.ind[c] > 0, ix, iy, iz);
-  out[c] = (R)v;
+  out[node_index(g, ix, iy, iz)] = (R)face_value(P, g, e, cf, ix, iy, iz);
 }
 
 // ------------------------------------------------------- initialization

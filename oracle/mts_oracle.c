@@ -801,8 +801,11 @@ int mts_oracle_init(const mts_oracle_problem *p, double t0, double *u) {
   return 0;
 }
 
-/* Douglas_ADI::Solve_2nd (douglas_adi.cc:66-1110) */
-static int step_adi(Prob *P, double dt, double t1, double *u, double *w[5]) {
+/* Douglas_ADI::Solve_2nd (douglas_adi.cc:66-1110).  stages (may be null):
+ * receives the stage-1 right-hand side with its boundary values (the input
+ * of the x sweep), u* and u** (the outputs of the x and y sweeps). */
+static int step_adi_stages(Prob *P, double dt, double t1, double *u, double *w[5],
+                           double *stages[3]) {
   double *uhs = w[0], *uhs2 = w[1], *uhss = w[2], *uh1 = w[3], *src = w[4];
   const double e = exp(-t1);
   const int nx = P->n[0], ny = P->n[1], nz = P->n[2];
@@ -820,13 +823,16 @@ static int step_adi(Prob *P, double dt, double t1, double *u, double *w[5]) {
         uhs[c] += dt * src[c];
       }
   set_bc(P, uhs, e);
+  if (stages && stages[0]) memcpy(stages[0], uhs, P->N * sizeof(double));
   if (sweep(P, 0, uhs, 1.0, dt, 0)) return -1;
+  if (stages && stages[1]) memcpy(stages[1], uhs, P->N * sizeof(double));
   douglas_T(P, 1, u, uhss, -dt);
   for (int iz = 1; iz < nz - 1; iz++)
     for (int iy = 1; iy < ny - 1; iy++)
       for (int ix = 1; ix < nx - 1; ix++) uhss[IDX(P, ix, iy, iz)] += uhs[IDX(P, ix, iy, iz)];
   set_bc(P, uhss, e);
   if (sweep(P, 1, uhss, 1.0, dt, 0)) return -1;
+  if (stages && stages[2]) memcpy(stages[2], uhss, P->N * sizeof(double));
   douglas_T(P, 2, u, uh1, -dt);
   for (int iz = 1; iz < nz - 1; iz++)
     for (int iy = 1; iy < ny - 1; iy++)
@@ -835,6 +841,10 @@ static int step_adi(Prob *P, double dt, double t1, double *u, double *w[5]) {
   if (sweep(P, 2, uh1, 1.0, dt, 0)) return -1;
   memcpy(u, uh1, P->N * sizeof(double));
   return 0;
+}
+
+static int step_adi(Prob *P, double dt, double t1, double *u, double *w[5]) {
+  return step_adi_stages(P, dt, t1, u, w, NULL);
 }
 
 /* LOD::Solve_2nd (lod.cc:44-205); fresh zeroed stage fields each step (lod.cc:53-71) */
@@ -913,6 +923,26 @@ int mts_oracle_run(const mts_oracle_problem *p, int method, double dt, double t0
     else rc = -1;
   }
   if (t_end) *t_end = t;
+  for (int i = 0; i < 5; i++) free(w[i]);
+  prob_free(&P);
+  return rc;
+}
+
+int mts_oracle_adi_stages(const mts_oracle_problem *p, double dt, double t0, double *u,
+                          double *rhs1, double *ustar, double *ustar2) {
+  Prob P;
+  if (prob_init(&P, p)) {
+    prob_free(&P);
+    return -1;
+  }
+  double *w[5];
+  int rc = 0;
+  for (int i = 0; i < 5; i++) {
+    w[i] = (double *)calloc(P.N, sizeof(double));
+    if (!w[i]) rc = -1;
+  }
+  double *stages[3] = {rhs1, ustar, ustar2};
+  if (rc == 0) rc = step_adi_stages(&P, dt, t0 + dt, u, w, stages);
   for (int i = 0; i < 5; i++) free(w[i]);
   prob_free(&P);
   return rc;

@@ -39,6 +39,13 @@ int mts_oracle_init(const mts_oracle_problem *p, double t0, double *u);
 int mts_oracle_run(const mts_oracle_problem *p, int method, double dt, double t0, int nsteps,
                    double *u, double *t_end);
 
+/* One Douglas-ADI step from t0 (u in/out) that also returns its stages:
+ * rhs1 the x sweep's right-hand side with boundary values (douglas_adi.cc:
+ * 93-359), ustar and ustar2 the outputs of the x and y sweeps (:377-514,
+ * :670-807); any of them may be null. */
+int mts_oracle_adi_stages(const mts_oracle_problem *p, double dt, double t0, double *u,
+                          double *rhs1, double *ustar, double *ustar2);
+
 /* Error norms (Error, douglas_adi.cc:1723-1770). */
 void mts_oracle_norms(const mts_oracle_problem *p, const double *u, double t, double *linf,
                       double *l2, int argmax[3]);

@@ -43,6 +43,8 @@ def lib():
         L.mts_oracle_run.argtypes = [P, C.c_int, C.c_double, C.c_double, C.c_int, C.c_void_p,
                                      C.POINTER(C.c_double)]
         L.mts_oracle_run.restype = C.c_int
+        L.mts_oracle_adi_stages.argtypes = [P, C.c_double, C.c_double] + [C.c_void_p] * 4
+        L.mts_oracle_adi_stages.restype = C.c_int
         L.mts_oracle_norms.argtypes = [P, C.c_void_p, C.c_double, C.POINTER(C.c_double),
                                        C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.mts_oracle_norms.restype = None
@@ -122,6 +124,16 @@ class OracleProblem:
             raise RuntimeError("oracle run failed (zero pivot or bad crossings)")
         return u, t_end.value
 
+    def adi_stages(self, dt, t0, u):
+        """One Douglas-ADI step from t0: (stage-1 right-hand side with boundary
+        values, u* after the x sweep, u** after the y sweep, u^{n+1})."""
+        u = np.ascontiguousarray(u, dtype=np.float64).copy()
+        r1, us, us2 = (np.empty(self.shape) for _ in range(3))
+        if lib().mts_oracle_adi_stages(C.byref(self._p), dt, t0, u.ctypes.data, r1.ctypes.data,
+                                       us.ctypes.data, us2.ctypes.data):
+            raise RuntimeError("oracle Douglas step failed")
+        return r1, us, us2, u
+
     def norms(self, u, t):
         u = np.ascontiguousarray(u, dtype=np.float64)
         li, l2 = C.c_double(), C.c_double()
@@ -176,8 +188,9 @@ def ref_available() -> bool:
 
 
 def run_reference(outdir, method, mib, n, dt, T, surface, beta, lo=-1.0, hi=1.0,
-                  max_steps=None, dump=True):
-    """Run oracle/_ref/mts_ref and parse its dumps (see oracle/ref_harness.cc)."""
+                  max_steps=None, dump=True, dump_steps=0):
+    """Run oracle/_ref/mts_ref and parse its dumps (see oracle/ref_harness.cc).
+    dump_steps: also return the fields after steps 1..dump_steps as u_steps."""
     os.makedirs(outdir, exist_ok=True)
     cmd = [REF_BIN, "--method", method, "--mib", str(mib), "--n", str(n), "--lo", repr(lo),
            "--hi", repr(hi), "--dt", repr(dt), "--T", repr(T), "--surface", surface,
@@ -186,6 +199,8 @@ def run_reference(outdir, method, mib, n, dt, T, surface, beta, lo=-1.0, hi=1.0,
         cmd += ["--max-steps", str(max_steps)]
     if not dump:
         cmd += ["--no-dump"]
+    if dump_steps:
+        cmd += ["--dump-steps", str(dump_steps)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     summ_path = os.path.join(outdir, "summary.txt")
     if proc.returncode != 0 or not os.path.exists(summ_path):
@@ -202,6 +217,10 @@ def run_reference(outdir, method, mib, n, dt, T, surface, beta, lo=-1.0, hi=1.0,
             p = os.path.join(outdir, name + ".bin")
             if os.path.exists(p):
                 out[name] = np.fromfile(p, dtype=np.float64).reshape(nn, nn, nn)
+        if dump_steps:
+            out["u_steps"] = np.stack([
+                np.fromfile(os.path.join(outdir, f"u_step{j}.bin"), dtype=np.float64)
+                .reshape(nn, nn, nn) for j in range(1, dump_steps + 1)])
         out["crossings"] = parse_crossings(os.path.join(outdir, "crossings.txt"))
         jp = os.path.join(outdir, "jumps.txt")
         if os.path.exists(jp) and os.path.getsize(jp):

@@ -12,6 +12,7 @@
 //                   Refresh_Jump (intersection.cc:2294-2490)
 //   u_step1.bin     float64 field after step 1  (x-fastest: (iz*ny+iy)*nx+ix)
 //   u_final.bin     float64 field after the last step
+//   u_step<j>.bin   with --dump-steps K: the field after step j, j = 1..K
 //   summary.txt     step count, final time, Lmax/L2/argmax (same loop as
 //                   Douglas_ADI::Error, douglas_adi.cc:1723-1770), timings
 //
@@ -235,6 +236,7 @@ struct Opts {
   string out = ".";
   int max_steps = -1;  // cap on steps (timing samples); -1 = all
   int dump = 1;
+  int dump_steps = 0;  // fields after steps 1..dump_steps (u_step<j>.bin)
 };
 
 static vector<double> parse_list(const string &s) {
@@ -253,7 +255,8 @@ static void usage() {
   fprintf(stderr,
           "mts_ref --method adi|lod-ie|lod-cn|ts --mib 1|2 --n N --lo L --hi H\n"
           "        --dt DT --T T --surface ellipsoid:a,b,c,r|torus:R,r|bumpy:R0,A,m,k\n"
-          "        --beta const:bin,bout|sincos:c,bout --out DIR [--max-steps K] [--no-dump]\n");
+          "        --beta const:bin,bout|sincos:c,bout --out DIR [--max-steps K] [--no-dump]\n"
+          "        [--dump-steps K]\n");
   exit(2);
 }
 
@@ -321,6 +324,7 @@ int main(int argc, char **argv) {
     else if (a == "--out") o.out = need();
     else if (a == "--max-steps") o.max_steps = atoi(need().c_str());
     else if (a == "--no-dump") o.dump = 0;
+    else if (a == "--dump-steps") o.dump_steps = atoi(need().c_str());
     else usage();
   }
 
@@ -406,6 +410,8 @@ int main(int argc, char **argv) {
       HEqExp eqn(t_now - dt, *beta), eqh(t_now - 0.5 * dt, *beta), eq1(t_now, *beta);
       ts->Solve_2nd(eqn, eqh, eq1, inter, uh, *beta);
     }
+    if (o.dump && j < o.dump_steps)
+      dump_field(o.out + "/u_step" + to_string(j + 1) + ".bin", uh, nx, ny, nz);
     if (j == 0) {
       first_step_s = chrono::duration<double>(chrono::steady_clock::now() - ts0).count();
       if (o.dump) {

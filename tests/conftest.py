@@ -62,6 +62,7 @@ def load_golden(name):
     z = np.load(os.path.join(GOLDEN_DIR, name + ".npz"), allow_pickle=False)
     g = {k: z[k] for k in z.files}
     g["meta"] = json.loads(str(g["meta"]))
+    g["u_step1"] = g["u_steps"][0]
     return g
 
 
@@ -98,3 +99,59 @@ def case_setup(mts, meta):
 
 def beta_kind(meta):
     return {"const": 0, "sincos": 1}[meta["beta"][0]]
+
+
+PARITY = 1e-12  # relative bar of the plain FP64 form against the reference
+
+
+def step_floor(oracle, op, method, dt, t, u_in, seed=0):
+    """Per-step parity bound for one step from u_in at time t: 1e-12, or,
+    where the step itself makes rounding matter more, four times the larger of
+      * the spread between two backward-stable elimination orders of the
+        step's line systems (the reference's Thomas and partial pivoting, both
+        in the oracle): any stable line solver other than the reference's own
+        Thomas order can differ from it by that much;
+      * the step's measured sensitivity to one rounding: the change of its
+        output when its input is perturbed by one ulp (a random sign per node),
+        which is what any implementation that rounds one intermediate
+        differently can move it by (large where explicit half-steps with a
+        strong contrast amplify, e.g. TS with beta 1000/1 at dt = h)."""
+    u_t, _ = op.run(method, dt, t, 1, u_in)
+    with oracle.line_solver("pivot"):
+        u_p, _ = op.run(method, dt, t, 1, u_in)
+    rng = np.random.default_rng(seed)
+    u_e = u_in * (1.0 + 2.0 ** -52 * rng.choice([-1.0, 1.0], np.shape(u_in)))
+    u_s, _ = op.run(method, dt, t, 1, u_e)
+    return max(PARITY, 4.0 * rel_err(u_p, u_t), 4.0 * rel_err(u_s, u_t))
+
+
+def run_amplification(op, method, dt, steps, seed=0):
+    """Measured amplification of a relative perturbation of the initial field
+    over the whole run (a random sign pattern of relative size 1e-9, mapped
+    through the oracle): how much a difference made at the first step can grow
+    by the end, whatever the solution's own growth."""
+    u0 = op.init(0.0)
+    uf, _ = op.run(method, dt, 0.0, steps, u0)
+    rng = np.random.default_rng(seed)
+    up = u0 * (1.0 + 1e-9 * rng.choice([-1.0, 1.0], u0.shape))
+    ufp, _ = op.run(method, dt, 0.0, steps, up)
+    return max(1.0, rel_err(ufp, uf) / rel_err(up, u0))
+
+
+def golden_oracle(mts, oracle, g):
+    """Oracle problem on a golden case's own crossings and weights."""
+    meta = g["meta"]
+    n = meta["n"] + 1
+    s, b = meta["surface"], meta["beta"]
+    phys = mts.physics(b[0], *b[1:])
+    st = mts.Setup(mts.grid(n), mts.surface(s[0], *s[1:]), phys, mib=meta["mib"])
+    return oracle.OracleProblem((n, n, n), -1.0, 1.0, st.indicator(), beta_kind(meta), b[1:],
+                                [golden_crossings(g, d) for d in range(3)])
+
+
+def step_times(dt, count):
+    """t^0 .. t^count accumulated as the reference's loop does (t += dt)."""
+    ts = [0.0]
+    for _ in range(count):
+        ts.append(ts[-1] + dt)
+    return ts

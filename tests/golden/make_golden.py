@@ -10,7 +10,9 @@ Each case becomes tests/golden/<name>.npz holding
   cr{d}_int    per crossing of direction d (0 x, 1 y, 2 z), in the product's
                line order: j, k, ID, left_loc, nw
   cr{d}_dbl    gamma, coord(3), normal(3), beta_in, beta_out, P(9), weil(9), weir(9)
-  u_step1      field after step 1, float64 (nz, ny, nx)
+  u_steps      fields after steps 1..K, float64 (K, nz, ny, nx), K = 3 (2 on the
+               33^3 grids), for per-step checks restarted from the reference's
+               own field (the tests read u_steps[0] as u_step1)
   u_final      field after the last step
   summary      steps, t_final, lmax, l2, argmax(3)
   jumps        [u] and u_dir per crossing after step 1's Refresh_Jump, in the
@@ -26,6 +28,7 @@ stored here.
 import json
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -41,34 +44,48 @@ CASES = []
 for _m in ("adi", "lod-ie", "lod-cn", "ts"):
     for _mib in (1, 2):
         CASES.append(dict(name=f"sphere16_{_m}_mib{_mib}", method=_m, mib=_mib, n=16, dt=0.125,
-                          T=0.5, surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1),
-                          keep_step1=True))
+                          T=0.5, surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1)))
 CASES += [
     # Douglas-ADI + MIB-L1, 33^3 nodes, sphere r=0.5 (pulled off the grid), 16 steps
     dict(name="cfg0_adi_mib1", method="adi", mib=1, n=32, dt=0.0625, T=1.0,
-         surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1), keep_step1=True),
+         surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1)),
     # configs[1] geometry and physics at 33^3 (the reference cannot set up 257^3: its
     # tangential-jump diagnostics exit for near-tangent crossings)
     dict(name="cfg1like32_lodcn_mib2", method="lod-cn", mib=2, n=32, dt=1 / 16, T=0.5,
-         surface=("ellipsoid", 0.7, 0.5, 0.4, E1), beta=("sincos", 10, 1), keep_step1=True),
+         surface=("ellipsoid", 0.7, 0.5, 0.4, E1), beta=("sincos", 10, 1)),
     # configs[2] physics (beta 1000/1, bumpy sphere) at 27^3
     dict(name="bumpy26_ts_mib2", method="ts", mib=2, n=26, dt=1 / 13, T=0.25,
-         surface=("bumpy", 0.5 * (1 - 2 ** -20), 0.1, 5, 3), beta=("const", 1000, 1),
-         keep_step1=True),
+         surface=("bumpy", 0.5 * (1 - 2 ** -20), 0.1, 5, 3), beta=("const", 1000, 1)),
     # configs[3]/[4] geometry family: torus, strong contrast either way
     dict(name="torus22_lodcn_mib2", method="lod-cn", mib=2, n=22, dt=1 / 11, T=0.5,
-         surface=("torus", 0.5, 0.2), beta=("const", 0.01, 1), keep_step1=True),
+         surface=("torus", 0.5, 0.2), beta=("const", 0.01, 1)),
     dict(name="torus22_lodie_mib1", method="lod-ie", mib=1, n=22, dt=1 / 11, T=0.5,
-         surface=("torus", 0.5, 0.17), beta=("const", 100, 1), keep_step1=True),
+         surface=("torus", 0.5, 0.17), beta=("const", 100, 1)),
     dict(name="ellipsoid20_adi_mib2", method="adi", mib=2, n=20, dt=0.1, T=0.5,
-         surface=("ellipsoid", 0.8, 0.6, 0.5, E1), beta=("const", 100, 1), keep_step1=True),
+         surface=("ellipsoid", 0.8, 0.6, 0.5, E1), beta=("const", 100, 1)),
+    # Douglas-ADI on interfaces with corner pairs (32 corner crossings along z on
+    # the torus, 8 along y and 8 along z on the configs[1] ellipsoid)
+    dict(name="torus22_adi_mib1", method="adi", mib=1, n=22, dt=1 / 11, T=0.5,
+         surface=("torus", 0.5, 0.17), beta=("const", 100, 1)),
+    dict(name="torus22_adi_mib2", method="adi", mib=2, n=22, dt=1 / 11, T=0.5,
+         surface=("torus", 0.5, 0.17), beta=("const", 0.01, 1)),
+    dict(name="cfg1like32_adi_mib1", method="adi", mib=1, n=32, dt=1 / 16, T=0.5,
+         surface=("ellipsoid", 0.7, 0.5, 0.4, E1), beta=("sincos", 10, 1)),
+    dict(name="cfg1like32_adi_mib2", method="adi", mib=2, n=32, dt=1 / 16, T=0.5,
+         surface=("ellipsoid", 0.7, 0.5, 0.4, E1), beta=("sincos", 10, 1)),
 ]
+
+
+def steps_kept(case):
+    """Per-step fields kept in the fixture (bounded by the fixture size)."""
+    return 2 if case["n"] >= 32 else 3
 
 
 def main():
     if not orc.ref_available():
         raise SystemExit("oracle/_ref/mts_ref is missing: run make -C oracle ref")
-    work = os.path.join(ROOT, "_work", "golden_runs")
+    tmp = tempfile.TemporaryDirectory(prefix="mts_golden_")
+    work = tmp.name
     for case in CASES:
         s = case["surface"]
         b = case["beta"]
@@ -77,7 +94,7 @@ def main():
         out = os.path.join(work, case["name"])
         try:
             ref = orc.run_reference(out, case["method"], case["mib"], case["n"], case["dt"],
-                                    case["T"], sstr, bstr)
+                                    case["T"], sstr, bstr, dump_steps=steps_kept(case))
         except RuntimeError as e:
             print(f"{case['name']}: skipped ({e})")
             continue
@@ -108,9 +125,9 @@ def main():
                                  cr["wl"], cr["wr"]], axis=1) if len(cr) else np.zeros((0, 36))
             arrays[f"cr{d}_int"] = ii.astype(np.int32)
             arrays[f"cr{d}_dbl"] = dd.astype(np.float64)
-        if case.get("keep_step1"):
-            arrays["u_step1"] = ref["u_step1"]
         arrays["u_final"] = ref["u_final"]
+        nk = min(steps_kept(case), int(ref["summary"]["steps"]))
+        arrays["u_steps"] = ref["u_steps"][:nk]
         sm = ref["summary"]
         arrays["summary"] = np.array([float(sm["steps"]), float(sm["t_final"]), float(sm["lmax"]),
                                       float(sm["l2"])] + [float(v) for v in sm["argmax"]])
@@ -120,6 +137,7 @@ def main():
         np.savez_compressed(path, **arrays)
         print(f"{case['name']}: {os.path.getsize(path) / 1024:.0f} KiB, steps {sm['steps']}, "
               f"lmax {sm['lmax']}")
+    tmp.cleanup()
 
 
 if __name__ == "__main__":

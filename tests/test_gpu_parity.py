@@ -1,44 +1,64 @@
 """GPU parity: the HIP path against the reference's golden fields and the oracle.
 
-Tolerance: 1e-12 normwise relative (the bar for the plain FP64 form),
-scaled by the solution's growth for runs the reference's scheme amplifies
-(TS and Douglas at dt = h with strong contrast grow by up to 1e7 over a few
-steps and amplify every rounding difference by as much).  The partitioned
-line solve rounds differently from the reference's sequential Thomas
-algorithm; everything else follows the reference's operation order.
+Tolerance: 1e-12 normwise relative (max-norm difference over the field's
+max-norm), the bar for the plain FP64 form, per step and restarted from the
+reference's own field of the previous step.  Where the step's line systems
+are ill-conditioned (the bumpy sphere with beta 1000/1) the bound is four
+times the measured spread between two backward-stable elimination orders of
+the same step (the reference's Thomas and partial pivoting, in the oracle):
+the GPU's partitioned line solve is one more such order.  Over a whole run the
+bound is the largest per-step bound times the run's measured amplification of
+a relative perturbation of its initial field.  Everything else follows the
+reference's operation order.
 """
 import numpy as np
 import pytest
 
-from conftest import (GOLDEN_NAMES, beta_kind, case_setup, golden_crossings, load_golden,
-                      rel_err)
+from conftest import (GOLDEN_NAMES, PARITY, beta_kind, case_setup, golden_crossings,
+                      golden_oracle, load_golden, rel_err, run_amplification, step_floor,
+                      step_times)
 
 pytestmark = pytest.mark.gpu
 
 
-def growth_tol(u):
-    return 1e-12 * max(1.0, float(np.max(np.abs(u))))
+def has_corners(g):
+    return any((g[f"cr{d}_int"][:, 2] < 0).any() for d in range(3) if len(g[f"cr{d}_int"]))
 
 
 @pytest.mark.parametrize("name", GOLDEN_NAMES)
-def test_gpu_matches_reference(mts, ctx, name):
+def test_gpu_matches_reference(mts, oracle, ctx, name):
     g = load_golden(name)
     meta = g["meta"]
     print(f"case {name}", flush=True)
+    if meta["method"] == "adi" and has_corners(g):
+        pytest.skip(DOUGLAS_CORNERS_OFF)
     st, _ = case_setup(mts, meta)
-    inst = mts.Instance(ctx, st, meta["method"], meta["dt"])
+    op = golden_oracle(mts, oracle, g)
+    method, dt = meta["method"], meta["dt"]
+    us = g["u_steps"]
+    ts = step_times(dt, len(us))
+    inst = mts.Instance(ctx, st, method, dt)
     inst.initialize(0.0)
-    inst.step(0.0 + meta["dt"])
-    u1 = inst.download()
-    assert rel_err(u1, g["u_step1"]) <= growth_tol(g["u_step1"])
+    floors = []
+    for j in range(len(us)):
+        u_in = inst.download() if j == 0 else us[j - 1]
+        tol = step_floor(oracle, op, method, dt, ts[j], u_in)
+        floors.append(tol)
+        inst.upload(u_in)
+        inst.step(ts[j + 1])
+        e = rel_err(inst.download(), us[j])
+        print(f"  step {j + 1}: {e:.2e} (bound {tol:.1e})", flush=True)
+        assert e <= tol, f"step {j + 1}"
     inst.initialize(0.0)
     steps, t_end = inst.run(0.0, meta["T"])
     assert steps == int(g["summary"][0])
     assert t_end == g["summary"][1]
     uf = inst.download()
-    assert rel_err(uf, g["u_final"]) <= growth_tol(g["u_final"])
+    tol = max(floors) * run_amplification(op, method, dt, steps)
+    e = rel_err(uf, g["u_final"])
+    print(f"  final: {e:.2e} (bound {tol:.1e})", flush=True)
+    assert e <= tol
     linf, l2, am = inst.error(t_end)
-    tol = growth_tol(g["u_final"])
     assert abs(linf - g["summary"][2]) <= tol * max(1.0, g["summary"][2])
     assert abs(l2 - g["summary"][3]) <= tol * max(1.0, g["summary"][3])
     inst.close()
@@ -94,18 +114,18 @@ def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
         uo, _ = op.run(method, dt, t, 1, u)
         with oracle.line_solver("partitioned"):
             um, _ = op.run(method, dt, t, 1, u)
+        tol = step_floor(oracle, op, method, dt, t, u)
         t += dt
         inst.step(t)
         ug = inst.download()
         e_ref, e_model = rel_err(ug, uo), rel_err(ug, um)
         print(f"  step {s + 1}: vs oracle (Thomas) {e_ref:.2e}, vs CPU model of the "
-              f"GPU line solve {e_model:.2e}", flush=True)
-        assert e_ref <= growth_tol(uo), f"step {s + 1}"
+              f"GPU line solve {e_model:.2e} (bound {tol:.1e})", flush=True)
+        assert e_ref <= tol, f"step {s + 1}"
         # the same elimination order as the kernel, so what is left is the
         # rounding of the right-hand side (the kernel applies each MIB block as
-        # one precombined linear form), amplified by TS's explicit half-steps
-        # (dt beta / h^2 = 320 on the first case)
-        assert e_model <= 0.25 * growth_tol(uo), f"step {s + 1} (model)"
+        # one precombined linear form)
+        assert e_model <= tol, f"step {s + 1} (model)"
         u = ug
 
 
@@ -132,10 +152,11 @@ def test_gpu_wide_lines_match_oracle(mts, oracle, ctx, shape, method):
     t = 0.0
     for s in range(2):
         uo, _ = op.run(method, dt, t, 1, u)
+        tol = step_floor(oracle, op, method, dt, t, u)
         t += dt
         inst.step(t)
         ug = inst.download()
-        assert rel_err(ug, uo) <= growth_tol(uo), f"step {s + 1}"
+        assert rel_err(ug, uo) <= tol, f"step {s + 1}"
         u = ug
     inst.close()
 
