@@ -9,13 +9,20 @@ u+ = e^-t cos x cos y cos z, dt = h = 1/128, T = 0.5 (64 steps), FP64.
 
 One step = one Solve_2nd: three line stages (x, y, z) and the boundary
 faces.  The field and the MIB table are resident in HBM before timing starts.
-The timed region is exactly K steps between two stream synchronizations; the
-per-stage HIP events that feed the roofline figure run inside it.
+The timed region is exactly K steps between two stream synchronizations, with
+nothing but the steps' launches in it.  The same K steps then run again with
+a pair of HIP events around every launch (on the instance's stream) for the
+per-stage times and the roofline kernel's average duration.
+
+--config adi257 is the Douglas-ADI target workload: MIB-L1, 257^3 nodes,
+sphere r = 0.5 (pulled off the grid node by 2^-20, relative), beta 10/1,
+dt = h = 1/128.
 
 Prints one JSON line.  --gpus above 1 is refused: this build runs on one GPU.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -34,10 +41,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=4)
-    ap.add_argument("--config", default="cfg1", choices=["cfg0", "cfg1", "cfg2"],
+    ap.add_argument("--config", default="cfg1", choices=["cfg0", "cfg1", "cfg2", "adi257"],
                     help="cfg1 (default): LOD-CN MIB-L2 257^3 ellipsoid; cfg0: Douglas-ADI "
                          "MIB-L1 33^3 sphere r=0.5, beta 10/1, dt=h, T=1; cfg2: TS MIB-L2 513^3 "
-                         "bumpy sphere, beta 1000/1, dt=h, T=0.25")
+                         "bumpy sphere, beta 1000/1, dt=h, T=0.25; adi257: Douglas-ADI MIB-L1 "
+                         "257^3 sphere r=0.5, beta 10/1, dt=h, T=0.5")
     ap.add_argument("--n", type=int, default=None, help="intervals per axis (overrides the config)")
     ap.add_argument("--method", default=None, choices=["adi", "lod-ie", "lod-cn", "ts"])
     ap.add_argument("--mib", type=int, default=None, choices=[1, 2])
@@ -87,6 +95,9 @@ def main():
         "cfg2": dict(n=512, method="ts", mib=2, T=0.25, beta=("const", 1000.0, 1.0),
                      surf=("bumpy", 0.5 * (1 - 2 ** -20), 0.1, 5, 3),
                      label="bumpy sphere r=0.5+0.1 sin5t cos3p beta-=1000 beta+=1"),
+        "adi257": dict(n=256, method="adi", mib=1, T=0.5, beta=("const", 10.0, 1.0),
+                       surf=("ellipsoid", 1.0, 1.0, 1.0, 0.5 * (1 - 2 ** -20)),
+                       label="sphere r=0.5 beta-=10 beta+=1"),
     }
     cf = cfgs[args.config]
     args.n = args.n or cf["n"]
@@ -113,10 +124,9 @@ def main():
         inst.step(t)
     inst.sync()
 
-    # timed run: K steps of the reference time loop (t += dt) from t0 = 0
+    # timed run: K steps of the reference time loop (t += dt) from t0 = 0, with
+    # nothing but the steps' own launches between the two synchronizations
     inst.initialize(0.0)
-    inst.profile(True)
-    inst.profile_reset()
     inst.sync()
     t = 0.0
     t_start = time.perf_counter()
@@ -125,9 +135,22 @@ def main():
         inst.step(t)
     inst.sync()
     elapsed = time.perf_counter() - t_start
+    linf, l2, _ = inst.error(t)
+    diverged = not (math.isfinite(l2) and linf < 1e6)
+
+    # the same K steps again with a pair of HIP events around every launch (on
+    # the instance's stream): per-stage times and the roofline kernel's average
+    inst.initialize(0.0)
+    inst.profile(True)
+    inst.profile_reset()
+    inst.sync()
+    tp = 0.0
+    for _ in range(args.steps):
+        tp += dt
+        inst.step(tp)
+    inst.sync()
     stages = inst.profile_read()
     inst.profile(False)
-    linf, l2, _ = inst.error(t)
 
     # the same K steps as one captured graph (the first run captures it)
     t_graph = (args.steps + 0.5) * dt
@@ -145,7 +168,7 @@ def main():
     # dominant stage: largest total time; algorithmic bytes = one FP64 read and
     # one FP64 write of every node (each line stage reads its input field once
     # and writes its output once; the Douglas y/z stages also read u^n)
-    per_node = {"adi_y": 24, "adi_z": 24}
+    per_node = {"adi_y": 24, "adi_z": 24, "adi_x": 16, "adi_rhs1": 16}
     line_stages = {k: v for k, v in stages.items() if k != "faces" and v[0] > 0}
     dom = max(line_stages, key=lambda k: line_stages[k][1])
     launches, total_ms = line_stages[dom]
@@ -164,7 +187,7 @@ def main():
 
     fp32 = None
     if args.fp32_form and args.method != "adi":
-        fp32 = fp32_form(mts, ctx, st, args, dt, nodes, value, linf)
+        fp32 = fp32_form(mts, ctx, st, args, dt, nodes, value, linf, diverged)
 
     cpu = None
     if args.cpu_baseline:
@@ -185,8 +208,10 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": f"{args.method.upper()} MIB-L{args.mib} {n}^3 {cf['label']}"
-                        f" dt=h={dt} T={T}",
+                        f" dt=h={dt}, {args.steps} steps from t=0 to t={t:.6g}"
+                        f" (the configuration's T={T} is {int(T / dt)} steps)",
             "config": args.config,
+            "diverged": diverged,
             "grid_points": nodes,
             "crossings": [int(info.n_cross[d]) for d in range(3)],
             "t_end": t,
@@ -215,9 +240,11 @@ def main():
     return 0
 
 
-def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
+def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf, fp64_diverged):
     """The same K steps in the FP32 form (FP32 fields and line solve, rows
-    formed in FP64), with its L-inf error against the exact solution."""
+    formed in FP64), with its L-inf error against the exact solution.  When
+    either form diverged (the L-inf ratio then compares overflow artefacts),
+    the ratios are null and "diverged" says so."""
     inst = mts.Instance(ctx, st, args.method, dt, precision="fp32")
     inst.initialize(0.0)
     t = 0.0
@@ -226,8 +253,6 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
         inst.step(t)
     inst.sync()
     inst.initialize(0.0)
-    inst.profile(True)
-    inst.profile_reset()
     inst.sync()
     t = 0.0
     t_start = time.perf_counter()
@@ -236,11 +261,21 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
         inst.step(t)
     inst.sync()
     elapsed = time.perf_counter() - t_start
+    linf, l2, _ = inst.error(t)
+    inst.initialize(0.0)
+    inst.profile(True)
+    inst.profile_reset()
+    inst.sync()
+    tp = 0.0
+    for _ in range(args.steps):
+        tp += dt
+        inst.step(tp)
+    inst.sync()
     stages = inst.profile_read()
     inst.profile(False)
-    linf, l2, _ = inst.error(t)
     inst.close()
     value = nodes * args.steps / elapsed
+    diverged = fp64_diverged or not (math.isfinite(l2) and linf < 1e6)
     return {
         "value": value,
         "unit": "grid-pt-steps/s",
@@ -248,8 +283,9 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf):
         "dtype": "f32 fields, rows and solve; f64 MIB rows, setup and norms",
         "linf_error": linf,
         "l2_error": l2,
-        "linf_vs_fp64": linf / fp64_linf,
-        "speedup_vs_fp64": value / fp64_value,
+        "diverged": diverged,
+        "linf_vs_fp64": None if diverged else linf / fp64_linf,
+        "speedup_vs_fp64": None if diverged else value / fp64_value,
         "stage_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in stages.items()},
     }
 
