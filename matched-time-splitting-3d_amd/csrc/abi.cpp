@@ -20,6 +20,7 @@
 #include "host_setup.hpp"
 #include "host_table.hpp"
 #include "kernels.hpp"
+#include "regular_solve.hpp"
 #include "stage_records.hpp"
 
 namespace {
@@ -139,6 +140,7 @@ struct mts_instance {
   mts_ctx *ctx = nullptr;
   int method = 0;
   int min_lanes = 16;  // lane-group lower bound of the line split (tuning)
+  int gen_lanes = 16;  // lanes per line of the general tiles' solve (tuning)
   double dt = 0.0, theta_dt = 0.0;
   mts::Grid grid;
   int px = 0;
@@ -165,6 +167,10 @@ struct mts_instance {
     auto it = recs.find(rec_key(d, flags));
     return it == recs.end() ? nullptr : it->second->p;
   }
+  // regular lines' solve tables per direction: [0] guarded, [1] unguarded
+  DevBuf<uint8_t> regt[3][2];
+  int reg_side[3] = {0, 0, 0};
+  DevBuf<int> tiles[3][2];  // per direction: regular tiles, the others (tile_lists)
   DevBuf<int> spec;
   int nspec = 0;
   DevBuf<double> part_max, part_sq;
@@ -195,7 +201,8 @@ struct mts_instance {
                part_sq.bytes() + part_idx.bytes();
     for (int d = 0; d < 3; d++)
       b += tab[d].bytes() + blk_ptr[d].bytes() + blk_hdr[d].bytes() + blk_val[d].bytes() +
-           cr_val[d].bytes() + seq[d].bytes() + cls0[d].bytes();
+           cr_val[d].bytes() + seq[d].bytes() + cls0[d].bytes() + regt[d][0].bytes() +
+           regt[d][1].bytes() + tiles[d][0].bytes() + tiles[d][1].bytes();
     for (auto &kv : recs) b += kv.second->bytes();
     return b;
   }
@@ -265,6 +272,7 @@ struct mts::InstancePlan {
   int method = 0;
   int precision = MTS_FP64;
   int min_lanes = 16;
+  int gen_lanes = 16;  // lanes per line of the general tiles' solve (tuning)
   double dt = 0.0, theta_dt = 0.0;
   mts::Grid grid;
   int px = 0;
@@ -272,8 +280,43 @@ struct mts::InstancePlan {
   mts::DeviceTableHost t;
   std::vector<int8_t> ind;                              // x-padded node classification
   std::vector<std::pair<int, std::vector<double>>> recs;  // (record key, stage records)
+  std::vector<uint8_t> reg[3][2];  // regular lines' solve tables (field type), guarded / unguarded
+  int reg_side[3] = {0, 0, 0};     // elements per side
+  std::vector<int> tiles[3][2];    // per direction: regular tiles, the others
   size_t device_bytes = 0;
 };
+
+namespace {
+// The regular lines' solve tables of direction d for an instance: the
+// outside side, then, when beta is constant inside, the inside side (lines
+// inside a variable beta are never regular), each reg_side_len elements of
+// the field type R, for the guarded solve and, when the split lets the
+// kernels use it, the unguarded one.  The rows' matrix coefficients are
+// formed exactly as the kernels form regular rows (in R).
+template <typename R>
+bool build_regular_tables(const mts::PhysConst &P, const mts::LinePart &q, double theta_dt, int d,
+                          std::vector<uint8_t> out[2], int &side_len) {
+  side_len = mts::reg_side_len(q.L, q.mmax);
+  for (int full = 0; full < 2; full++) {
+    out[full].clear();
+    if (full && q.m != q.mmax) continue;
+    std::vector<double> tab;
+    for (int side = 1; side >= 0; side--) {
+      if (side == 0 && P.beta_kind != 0) continue;
+      const R rt = (R)theta_dt;
+      const R w0 = (R)P.vdc[d][side][0], w1 = (R)P.vdc[d][side][1], w2 = (R)P.vdc[d][side][2];
+      const R a = -rt * w0, b = R(1) - rt * w1, c = -rt * w2;
+      if (!mts::regular_factors(q, q.mmax, full != 0, (double)a, (double)b, (double)c, tab))
+        return false;
+    }
+    std::vector<R> tr(tab.begin(), tab.end());
+    // whole 16-byte units (the kernels stage the tables with 16-byte loads)
+    out[full].assign((tr.size() * sizeof(R) + 15) / 16 * 16, 0);
+    std::memcpy(out[full].data(), tr.data(), tr.size() * sizeof(R));
+  }
+  return true;
+}
+}  // namespace
 
 void mts::PlanDeleter::operator()(mts::InstancePlan *p) const { delete p; }
 
@@ -354,6 +397,18 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       if (!rerr.empty()) return fail(MTS_E_ARG, rerr);
       p->recs.push_back({mts_instance::rec_key(kd.first, kd.second), std::move(rec)});
     }
+    for (int d = 0; d < 3; d++) {
+      mts::LinePart q;
+      mts::make_line_part(g.n[d], q, p->min_lanes);
+      const bool ok = precision == MTS_FP32
+                          ? build_regular_tables<float>(t.phys, q, p->theta_dt, d, p->reg[d], p->reg_side[d])
+                          : build_regular_tables<double>(t.phys, q, p->theta_dt, d, p->reg[d], p->reg_side[d]);
+      if (!ok) return fail(MTS_E_ARG, "zero pivot in the regular lines' line solve");
+      const int nj = d == 0 ? g.n[1] : g.n[0];
+      const int nk = d == 2 ? g.n[1] : g.n[2];
+      mts::stage_tiles(t.blk_ptr[d].data(), t.cls0[d].data(), t.phys.beta_kind == 0, g.n[d], nj,
+                       nk, p->min_lanes, p->gen_lanes, p->tiles[d][0], p->tiles[d][1]);
+    }
     // exactly what instance_from_plan allocates
     size_t b = p->field_elems * ((precision == MTS_FP32 ? sizeof(float) : sizeof(double)) +
                                  (method == MTS_ADI ? sizeof(double) : 0) + sizeof(int8_t));
@@ -363,6 +418,9 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
            t.cr_val[d].size() * sizeof(double) + (t.n_seq[d] > 0 ? t.seq[d].size() : 0) +
            t.cls0[d].size();
     for (auto &r : p->recs) b += r.second.size() * sizeof(double);
+    for (int d = 0; d < 3; d++)
+      b += p->reg[d][0].size() + p->reg[d][1].size() +
+           (p->tiles[d][0].size() + p->tiles[d][1].size()) * sizeof(int);
     b += t.spec.size() * sizeof(int);
     mts::DevGeom dg;
     for (int a = 0; a < 3; a++) dg.n[a] = g.n[a];
@@ -385,7 +443,8 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
     mts::make_line_part(g.n[d], q, p.min_lanes);
     const size_t need =
         ((size_t)mts::tile_lines(q.L) * (q.lds_len * 2 + mts::kWtLen) + 2 * (size_t)g.n[2]) *
-        sizeof(double);
+            sizeof(double) +
+        16 + 2 * (size_t)mts::reg_side_len(q.L, q.mmax) * sizeof(double);  // (the larger kernel)
     if (need > ctx->prop.sharedMemPerBlock)
       return fail(MTS_E_ARG, "line too long for the workgroup LDS of this device");
   }
@@ -398,10 +457,12 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   in->theta_dt = p.theta_dt;
   in->grid = g;
   in->min_lanes = p.min_lanes;
+  in->gen_lanes = p.gen_lanes;
   in->px = p.px;
   in->field_elems = p.field_elems;
   MTS_HIP(hipStreamCreateWithFlags(&in->sh.s, hipStreamNonBlocking));
   in->stream = in->sh.s;
+
   const mts::DeviceTableHost &t = p.t;
   in->phys = t.phys;
 
@@ -431,6 +492,13 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
     MTS_HIP(in->cr_val[d].upload(t.cr_val[d].data(), t.cr_val[d].size(), st));
     if (t.n_seq[d] > 0) MTS_HIP(in->seq[d].upload(t.seq[d].data(), t.seq[d].size(), st));
     MTS_HIP(in->cls0[d].upload(t.cls0[d].data(), t.cls0[d].size(), st));
+  }
+  for (int d = 0; d < 3; d++) {
+    in->reg_side[d] = p.reg_side[d];
+    for (int f = 0; f < 2; f++) {
+      MTS_HIP(in->regt[d][f].upload(p.reg[d][f].data(), p.reg[d][f].size(), st));
+      MTS_HIP(in->tiles[d][f].upload(p.tiles[d][f].data(), p.tiles[d][f].size(), st));
+    }
   }
   for (auto &r : p.recs) {
     std::unique_ptr<DevBuf<double>> b(new DevBuf<double>());
@@ -804,6 +872,19 @@ static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::L
   mts::LineArgs a = args;
   a.mib.rec = in->rec_for(dir, flags);
   if (!a.mib.rec && in->mib[dir].hdr) return hipErrorInvalidValue;
+  const bool solve = (flags & mts::kFlagSolve) != 0;
+  a.reg[0] = solve ? in->regt[dir][0].p : nullptr;
+  a.reg[1] = solve ? in->regt[dir][1].p : nullptr;
+  a.reg_side = in->reg_side[dir];
+  a.reg_sides = in->phys.beta_kind == 0 ? 2 : 1;
+  // the stage's tiles split into the regular and the other ones (an empty
+  // list has no buffer and a count of 0)
+  a.split = solve ? 1 : 0;
+  a.gen_lanes = in->gen_lanes;
+  for (int c = 0; c < 2; c++) {
+    a.tiles[c] = solve ? in->tiles[dir][c].p : nullptr;
+    a.ntiles[c] = solve ? (int)in->tiles[dir][c].n : 0;
+  }
   return mts::launch_line(dir, flags, in->f32(), a, st);
 }
 

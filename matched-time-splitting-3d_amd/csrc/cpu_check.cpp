@@ -17,6 +17,15 @@
 //   mts_cpucheck <method adi|lod-ie|lod-cn|ts> <mib 1|2> <nodes per axis>
 //                <surface ellipsoid|torus|bumpy> <p0> <p1> [p2 p3]
 //                <beta const|sincos> <b0> <b1> [--dump DIR]
+//
+//   mts_cpucheck solve <nodes> <min lanes> <k>
+//     the regular lines' right-hand-side-only solve (regular_solve.hpp) on a
+//     line of <nodes> nodes with rows (-k, 1 + 2k, -k), every lane of the
+//     group emulated phase by phase, against a Thomas solve of the same line;
+//     prints the largest relative difference of the full and the guarded
+//     variant.
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +37,7 @@
 #include "device_core.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
+#include "regular_solve.hpp"
 #include "stage_records.hpp"
 
 using namespace mts;
@@ -44,6 +54,7 @@ struct Emu {
   DevGeom g;
   DevMib mib[3];
   PhysConst phys;
+  long long reg_rows_checked = 0;  // regular rows whose right-hand sides were compared
 };
 
 // One pair of a thread's walk against the line and row it stands for:
@@ -79,7 +90,6 @@ void emulate_line(Emu &E, const LineArgs &p) {
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;
   constexpr int T = tile_lines(L);
-  (void)E;
   const LinePart &q = p.q;
   const int n = p.g.n[DIR];
   const int nj = LineMap<DIR>::nj(p.g), nk = LineMap<DIR>::nk(p.g);
@@ -158,6 +168,16 @@ void emulate_line(Emu &E, const LineArgs &p) {
             lb[kseg * MMAX + r] = b[r];
             lc[kseg * MMAX + r] = c[r];
             ld[kseg * MMAX + r] = d[r];
+          }
+          if (SOLVE && regular_line(p.P, lm, valid)) {
+            // the regular solve's right-hand side is the general one, bit for bit
+            double dr[MMAX];
+            const LineSource<DIR> lsrc(p.P, p.g, j, k, DIR == 2 ? zt.data() : nullptr);
+            regular_rhs<DIR, MMAX, FLAGS>(p, j, k, lm.out0, tA.get() + l * q.lds_len,
+                                          tB.get() + l * q.lds_len, wt.get() + l * kWtLen, lsrc,
+                                          sg, dr);
+            if (std::memcmp(dr, d, sizeof(dr)) != 0) std::abort();
+            E.reg_rows_checked += sg.m;
           }
         }
         if (!valid) continue;
@@ -248,6 +268,123 @@ void line(Emu &E, int dir, const LineArgs &args) {
   else std::abort();
 }
 
+// The regular lines' right-hand-side-only solve of one line, every lane of
+// the group emulated: the per-lane phases are the kernels' own functions, the
+// cross-lane shifts are done here.  Returns the largest difference from x.
+template <int MMAX, bool FULL>
+double regular_line(const LinePart &q, const std::vector<double> &tab, const std::vector<double> &d,
+                    const std::vector<double> &x) {
+  const int L = q.L, n = q.n;
+  std::vector<std::array<double, MMAX>> dl((size_t)L);
+  std::vector<double> D1((size_t)L), D2((size_t)L), ed((size_t)L), xe((size_t)L);
+  std::vector<int> ms((size_t)L);
+  for (int k = 0; k < L; k++) {
+    const LaneSeg sg = lane_seg(q, k, true);
+    ms[(size_t)k] = sg.m;
+    double row[MMAX];
+    for (int r = 0; r < MMAX; r++) row[r] = r < sg.m ? d[(size_t)(sg.s + r)] : 0.0;
+    const double *rows = tab.data() + reg_rows_off(k == q.p - 1 ? 1 : 0, MMAX);
+    const double *sc = tab.data() + reg_scal_off(k, MMAX);
+    reg_forward<MMAX, FULL>(rows, sc, row, sg.m, D1[(size_t)k], D2[(size_t)k]);
+    for (int r = 0; r < MMAX; r++) dl[(size_t)k][(size_t)r] = row[r];
+  }
+  for (int k = 0; k < L; k++) {
+    const double *sc = tab.data() + reg_scal_off(k, MMAX);
+    const double D1n = k + 1 < L ? D1[(size_t)k + 1] : 0.0;
+    ed[(size_t)k] = reg_fold(sc, FULL ? MMAX : ms[(size_t)k], D1[(size_t)k], D2[(size_t)k], D1n);
+  }
+  for (int lv = 0, s = 1; s < L; lv++, s <<= 1) {
+    std::vector<double> nd((size_t)L);
+    for (int k = 0; k < L; k++) {
+      const double *sc = tab.data() + reg_scal_off(k, MMAX);
+      const double dm = k >= s ? ed[(size_t)(k - s)] : 0.0;
+      const double dp = k + s < L ? ed[(size_t)(k + s)] : 0.0;
+      nd[(size_t)k] = reg_level(sc, lv, ed[(size_t)k], dm, dp);
+    }
+    ed = nd;
+  }
+  for (int k = 0; k < L; k++) xe[(size_t)k] = ed[(size_t)k] * tab[(size_t)reg_scal_off(k, MMAX) + kRegQ];
+  double err = 0.0, mx = 0.0;
+  for (int k = 0; k < L; k++) {
+    const LaneSeg sg = lane_seg(q, k, true);
+    const double *rows = tab.data() + reg_rows_off(k == q.p - 1 ? 1 : 0, MMAX);
+    const double *sc = tab.data() + reg_scal_off(k, MMAX);
+    double row[MMAX];
+    for (int r = 0; r < MMAX; r++) row[r] = dl[(size_t)k][(size_t)r];
+    const double xprev = k >= 1 ? xe[(size_t)k - 1] : 0.0;
+    reg_back<MMAX, FULL>(rows, sc, row, sg.m, D1[(size_t)k], xe[(size_t)k], xprev);
+    for (int r = 0; r < sg.m; r++) {
+      err = std::max(err, std::fabs(row[r] - x[(size_t)(sg.s + r)]));
+      mx = std::max(mx, std::fabs(x[(size_t)(sg.s + r)]));
+    }
+  }
+  (void)n;
+  return err / mx;
+}
+
+// The Douglas stage-1 right-hand side at one node written with each load
+// where its value is used (the kernel's node function loads everything up
+// front); the two must agree bit for bit.
+double rhs1_plain(const Rhs1Args &p, int ix, int iy, int iz) {
+  const DevGeom &g = p.g;
+  const size_t c = node_index(g, ix, iy, iz);
+  if (ix == 0 || iy == 0 || iz == 0 || ix == g.n[0] - 1 || iy == g.n[1] - 1 || iz == g.n[2] - 1)
+    return outer_u(g, p.e, ix, iy, iz);
+  const bool out = g.ind[c] > 0;
+  const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
+  double vy[3], vz[3];
+  op_weights(p.P, g, 1, out, ix, iy, iz, vy);
+  op_weights(p.P, g, 2, out, ix, iy, iz, vz);
+  const double u = p.u[c];
+  double sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
+  double sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+  double r = p.dt * sum_y;
+  r += p.dt * sum_z;
+  r += u;
+  r += p.dt * source(p.P, g, p.e, out, ix, iy, iz);
+  return r;
+}
+
+int check_solve(int n, int min_lanes, double kk) {
+  LinePart q;
+  if (!make_line_part(n, q, min_lanes)) return 2;
+  const double a = -kk, b = 1.0 + 2.0 * kk, c = -kk;
+  // a right-hand side on rows 1 .. n-2 with the boundary couplings moved in
+  std::vector<double> d((size_t)n, 0.0), x((size_t)n, 0.0);
+  unsigned s = 12345u + (unsigned)n;
+  for (int i = 1; i <= n - 2; i++) {
+    s = s * 1103515245u + 12345u;
+    d[(size_t)i] = ((s >> 8) & 0xffff) / 65536.0 - 0.5;
+  }
+  // Thomas over rows 1 .. n-2
+  {
+    std::vector<double> gam((size_t)n);
+    double bet = b;
+    x[1] = d[1] / bet;
+    for (int i = 2; i <= n - 2; i++) {
+      gam[(size_t)i] = c / bet;
+      bet = b - a * gam[(size_t)i];
+      x[(size_t)i] = (d[(size_t)i] - a * x[(size_t)i - 1]) / bet;
+    }
+    for (int i = n - 3; i >= 1; i--) x[(size_t)i] -= gam[(size_t)i + 1] * x[(size_t)i + 1];
+  }
+  double worst[2] = {-1.0, -1.0};
+  for (int f = 0; f < 2; f++) {
+    const bool full = f == 1;
+    if (full && q.m != q.mmax) continue;
+    std::vector<double> tab;
+    if (!regular_factors(q, q.mmax, full, a, b, c, tab)) return 4;
+    double e = 0.0;
+    if (q.mmax == 4) e = full ? regular_line<4, true>(q, tab, d, x) : regular_line<4, false>(q, tab, d, x);
+    else if (q.mmax == 8) e = full ? regular_line<8, true>(q, tab, d, x) : regular_line<8, false>(q, tab, d, x);
+    else e = full ? regular_line<16, true>(q, tab, d, x) : regular_line<16, false>(q, tab, d, x);
+    worst[f] = e;
+  }
+  std::printf("ok solve n %d L %d mmax %d m %d mlast %d p %d guarded %.3e full %.3e\n", n, q.L,
+              q.mmax, q.m, q.mlast, q.p, worst[0], worst[1]);
+  return 0;
+}
+
 // Field (x-pitched) to nx*ny*nz, x fastest, written as raw float64.
 void dump_field(const std::string &path, const std::vector<double> &f, const int nn[3], int px) {
   std::vector<double> out((size_t)nn[0] * nn[1] * nn[2]);
@@ -274,6 +411,8 @@ void faces(const Emu &E, std::vector<double> &u, double e, double cf) {
 }  // namespace
 
 int main(int argc, char **argv) {
+  if (argc == 5 && std::string(argv[1]) == "solve")
+    return check_solve(std::atoi(argv[2]), std::atoi(argv[3]), std::atof(argv[4]));
   std::string dump_dir;
   if (argc > 2 && std::string(argv[argc - 2]) == "--dump") {
     dump_dir = argv[argc - 1];
@@ -353,6 +492,49 @@ int main(int argc, char **argv) {
     E.mib[d].seq = E.t.n_seq[d] > 0 ? E.t.seq[d].data() : nullptr;
     E.mib[d].cls0 = E.t.cls0[d].data();
   }
+  // the solve stages' tile lists: every interior line in exactly one tile,
+  // the regular tiles holding regular lines only
+  {
+    int ml = 16;
+    if (const char *ev = std::getenv("MTS_LINE_LANES")) ml = std::atoi(ev);
+    for (int d = 0; d < 3; d++) {
+      const int nj = d == 0 ? nn[1] : nn[0], nk = d == 2 ? nn[1] : nn[2];
+      std::vector<int> lr, lg;
+      if (!stage_tiles(E.t.blk_ptr[d].data(), E.t.cls0[d].data(), E.t.phys.beta_kind == 0, nn[d], nj,
+                       nk, ml, 32, lr, lg))
+        std::abort();
+      LinePart qr, qg;
+      make_line_part(nn[d], qr, ml);
+      make_line_part(nn[d], qg, std::max(ml, 32));
+      std::vector<int> cover((size_t)nj * nk, 0);
+      for (int c = 0; c < 2; c++) {
+        const int T = tile_lines(c == 0 ? qr.L : qg.L);
+        for (int e : c == 0 ? lr : lg) {
+          const int bx = tile_bx(e), k = tile_k(e);
+          if (k < 1 || k > nk - 2 || bx * T >= nj) std::abort();
+          for (int j = bx * T; j < bx * T + T && j < nj; j++) {
+            if (j < 1 || j > nj - 2) continue;
+            const int lid = k * nj + j;
+            cover[(size_t)lid]++;
+            const bool reg = E.t.blk_ptr[d][lid] == E.t.blk_ptr[d][lid + 1] &&
+                             (E.t.cls0[d][lid] || E.t.phys.beta_kind == 0);
+            if (c == 0 && !reg) {
+              std::printf("dir %d: irregular line %d in a regular tile\n", d, lid);
+              return 1;
+            }
+          }
+        }
+      }
+      for (int k = 1; k <= nk - 2; k++)
+        for (int j = 1; j <= nj - 2; j++)
+          if (cover[(size_t)k * nj + j] != 1) {
+            std::printf("dir %d: line (%d, %d) in %d tiles\n", d, j, k, cover[(size_t)k * nj + j]);
+            return 1;
+          }
+      std::printf("dir %d tiles: %zu regular (L %d), %zu general (L %d)\n", d, lr.size(), qr.L,
+                  lg.size(), qg.L);
+    }
+  }
   // time factors: the first step from t = 0 (mts_step with t_next = dt)
   const double t1 = dt;
   const double e1 = std::exp(-t1), en = std::exp(-(t1 - dt)), eh = std::exp(-(t1 - 0.5 * dt));
@@ -378,7 +560,14 @@ int main(int argc, char **argv) {
     Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, e1};
     for (int iz = 0; iz < nn[2]; iz++)
       for (int iy = 0; iy < nn[1]; iy++)
-        for (int ix = 0; ix < nn[0]; ix++) rhs1_node(r1, ix, iy, iz);
+        for (int ix = 0; ix < nn[0]; ix++) {
+          rhs1_node(r1, ix, iy, iz);
+          const double v = rhs1_plain(r1, ix, iy, iz);
+          if (std::memcmp(&v, &E.r[node_index(E.g, ix, iy, iz)], sizeof v) != 0) {
+            std::printf("rhs1 node (%d, %d, %d) differs from the plain form\n", ix, iy, iz);
+            return 1;
+          }
+        }
     SpecArgs sp;
     sp.g = E.g;
     sp.P = E.t.phys;
@@ -457,7 +646,8 @@ int main(int argc, char **argv) {
     return 2;
   }
   if (dump) dump_field(dump_dir + "/u1.bin", E.u, nn, E.px);
-  std::printf("ok %s mib%d %dx%dx%d crossings %zu %zu %zu\n", method.c_str(), mib, nn[0], nn[1], nn[2],
-              res.cross[0].size(), res.cross[1].size(), res.cross[2].size());
+  std::printf("ok %s mib%d %dx%dx%d crossings %zu %zu %zu regular rows %lld\n", method.c_str(), mib,
+              nn[0], nn[1], nn[2], res.cross[0].size(), res.cross[1].size(), res.cross[2].size(),
+              E.reg_rows_checked);
   return 0;
 }

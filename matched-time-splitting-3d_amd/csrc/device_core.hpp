@@ -43,29 +43,75 @@ MTS_HD size_t node_index(const DevGeom &g, int ix, int iy, int iz) {
 }
 
 // ------------------------------------------------------------- physics
-// u+ = e cos x cos y cos z (the boundary value and the outer exact solution)
-MTS_HD double outer_u(const DevGeom &g, double e, int ix, int iy, int iz) {
-  return e * tabv(g, 0, kTabCos, ix) * tabv(g, 1, kTabCos, iy) * tabv(g, 2, kTabCos, iz);
+// The per-node physics below reads the trigonometric tables through an
+// accessor tv(a, kind) (the entry of axis a at the node), so that one text of
+// each formula serves both callers that load entries where they are used and
+// callers that load them all up front.
+template <class TV>
+MTS_HD double outer_with(double e, const TV &tv) {
+  return e * tv(0, kTabCos) * tv(1, kTabCos) * tv(2, kTabCos);
 }
-// u- = e sin x sin y sin z
-MTS_HD double inner_u(const DevGeom &g, double e, int ix, int iy, int iz) {
-  return e * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabSin, iy) * tabv(g, 2, kTabSin, iz);
-}
-
 // f = (3 beta - 1) u - grad(beta).grad(u) on the node's side.
-MTS_HD double source(const PhysConst &P, const DevGeom &g, double e,
-                                         bool outside, int ix, int iy, int iz) {
-  if (outside) return P.f_out * outer_u(g, e, ix, iy, iz);
-  double sx = tabv(g, 0, kTabSin, ix), sy = tabv(g, 1, kTabSin, iy), sz = tabv(g, 2, kTabSin, iz);
+template <class TV>
+MTS_HD double source_with(const PhysConst &P, double e, bool outside, const TV &tv) {
+  if (outside) return P.f_out * outer_with(e, tv);
+  double sx = tv(0, kTabSin), sy = tv(1, kTabSin), sz = tv(2, kTabSin);
   double u = e * sx * sy * sz;
   if (P.beta_kind == 0) return P.f_in * u;
-  double cx = tabv(g, 0, kTabCos, ix), cy = tabv(g, 1, kTabCos, iy);
+  double cx = tv(0, kTabCos), cy = tv(1, kTabCos);
   double b = P.b_in * (1.0 + 0.5 * sx * cy);
   double bx = 0.5 * P.b_in * cx * cy;
   double by = -0.5 * P.b_in * sx * sy;
   double ux = e * cx * sy * sz;
   double uy = e * sx * cy * sz;
   return (3.0 * b - 1.0) * u - bx * ux - by * uy;
+}
+// Operator weights beta(x -+ h/2)/h^2 along axis d on the node's side.
+template <class TV>
+MTS_HD void op_weights_with(const PhysConst &P, int d, bool outside, const TV &tv, double vd[3]) {
+  if (outside || P.beta_kind == 0) {
+    const int sd = outside ? 1 : 0;
+    vd[0] = P.vdc[d][sd][0];
+    vd[1] = P.vdc[d][sd][1];
+    vd[2] = P.vdc[d][sd][2];
+    return;
+  }
+  double b1, b2;
+  if (d == 0) {
+    double cy = tv(1, kTabCos);
+    b1 = P.b_in * (1.0 + 0.5 * tv(0, kTabSinM) * cy);
+    b2 = P.b_in * (1.0 + 0.5 * tv(0, kTabSinP) * cy);
+  } else if (d == 1) {
+    double sx = tv(0, kTabSin);
+    b1 = P.b_in * (1.0 + 0.5 * sx * tv(1, kTabCosM));
+    b2 = P.b_in * (1.0 + 0.5 * sx * tv(1, kTabCosP));
+  } else {
+    b1 = P.b_in * (1.0 + 0.5 * tv(0, kTabSin) * tv(1, kTabCos));
+    b2 = b1;
+  }
+  vd[0] = b1 * P.ihh[d];
+  vd[1] = -(b1 + b2) * P.ihh[d];
+  vd[2] = b2 * P.ihh[d];
+}
+// The accessor that loads each entry where it is used.
+struct TabAt {
+  const DevGeom &g;
+  int i[3];
+  MTS_HD double operator()(int a, int kind) const { return tabv(g, a, kind, i[a]); }
+};
+
+// u+ = e cos x cos y cos z (the boundary value and the outer exact solution)
+MTS_HD double outer_u(const DevGeom &g, double e, int ix, int iy, int iz) {
+  return outer_with(e, TabAt{g, {ix, iy, iz}});
+}
+// u- = e sin x sin y sin z
+MTS_HD double inner_u(const DevGeom &g, double e, int ix, int iy, int iz) {
+  return e * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabSin, iy) * tabv(g, 2, kTabSin, iz);
+}
+
+MTS_HD double source(const PhysConst &P, const DevGeom &g, double e,
+                                         bool outside, int ix, int iy, int iz) {
+  return source_with(P, e, outside, TabAt{g, {ix, iy, iz}});
 }
 
 // Value of boundary node (ix, iy, iz): the exact outer solution at e, plus
@@ -77,33 +123,9 @@ MTS_HD double face_value(const PhysConst &P, const DevGeom &g, double e, double 
   return v;
 }
 
-// Operator weights beta(x -+ h/2)/h^2 along axis d on the node's side.
 MTS_HD void op_weights(const PhysConst &P, const DevGeom &g, int d,
                                            bool outside, int ix, int iy, int iz, double vd[3]) {
-  (void)iz;  // beta depends on x and y only
-  if (outside || P.beta_kind == 0) {
-    const int sd = outside ? 1 : 0;
-    vd[0] = P.vdc[d][sd][0];
-    vd[1] = P.vdc[d][sd][1];
-    vd[2] = P.vdc[d][sd][2];
-    return;
-  }
-  double b1, b2;
-  if (d == 0) {
-    double cy = tabv(g, 1, kTabCos, iy);
-    b1 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSinM, ix) * cy);
-    b2 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSinP, ix) * cy);
-  } else if (d == 1) {
-    double sx = tabv(g, 0, kTabSin, ix);
-    b1 = P.b_in * (1.0 + 0.5 * sx * tabv(g, 1, kTabCosM, iy));
-    b2 = P.b_in * (1.0 + 0.5 * sx * tabv(g, 1, kTabCosP, iy));
-  } else {
-    b1 = P.b_in * (1.0 + 0.5 * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabCos, iy));
-    b2 = b1;
-  }
-  vd[0] = b1 * P.ihh[d];
-  vd[1] = -(b1 + b2) * P.ihh[d];
-  vd[2] = b2 * P.ihh[d];
+  op_weights_with(P, d, outside, TabAt{g, {ix, iy, iz}}, vd);
 }
 
 // [u] and u_dir = P^{-1}[d][0] n.[beta grad u] at one crossing
@@ -230,26 +252,46 @@ MTS_HD double block_sum(const DevMib &mib, int bidx, int rr,
 
 // Douglas stage-1 right-hand side at one node (douglas_adi.cc:93-359):
 // R = dt L_y u + dt L_z u + u + dt f, boundary nodes get the boundary value.
+// Every load is issued before the first is used (a boundary node reads itself
+// in place of its neighbours), so a node costs one memory round trip.
+struct Rhs1Trig {
+  double s[3], c[3], cym, cyp;
+  MTS_HD double operator()(int a, int kind) const {
+    return kind == kTabSin ? s[a] : kind == kTabCos ? c[a] : kind == kTabCosM ? cym : cyp;
+  }
+};
 MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
   const DevGeom &g = p.g;
   const size_t c = node_index(g, ix, iy, iz);
-  if (ix == 0 || iy == 0 || iz == 0 || ix == g.n[0] - 1 || iy == g.n[1] - 1 ||
-      iz == g.n[2] - 1) {
-    p.R[c] = outer_u(g, p.e, ix, iy, iz);
+  const bool bnd = ix == 0 || iy == 0 || iz == 0 || ix == g.n[0] - 1 || iy == g.n[1] - 1 ||
+                   iz == g.n[2] - 1;
+  const size_t sy = bnd ? 0 : (size_t)g.px, sz = bnd ? 0 : (size_t)g.n[1] * g.px;
+  const signed char ind = g.ind[c];
+  const double u = p.u[c];
+  const double uym = p.u[c - sy], uyp = p.u[c + sy], uzm = p.u[c - sz], uzp = p.u[c + sz];
+  Rhs1Trig t;
+  t.s[0] = tabv(g, 0, kTabSin, ix);
+  t.s[1] = tabv(g, 1, kTabSin, iy);
+  t.s[2] = tabv(g, 2, kTabSin, iz);
+  t.c[0] = tabv(g, 0, kTabCos, ix);
+  t.c[1] = tabv(g, 1, kTabCos, iy);
+  t.c[2] = tabv(g, 2, kTabCos, iz);
+  t.cym = tabv(g, 1, kTabCosM, iy);
+  t.cyp = tabv(g, 1, kTabCosP, iy);
+  if (bnd) {
+    p.R[c] = outer_with(p.e, t);
     return;
   }
-  const bool out = g.ind[c] > 0;
-  const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
+  const bool out = ind > 0;
   double vy[3], vz[3];
-  op_weights(p.P, g, 1, out, ix, iy, iz, vy);
-  op_weights(p.P, g, 2, out, ix, iy, iz, vz);
-  const double u = p.u[c];
-  double sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
-  double sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+  op_weights_with(p.P, 1, out, t, vy);
+  op_weights_with(p.P, 2, out, t, vz);
+  double sum_y = uym * vy[0] + u * vy[1] + uyp * vy[2];
+  double sum_z = uzm * vz[0] + u * vz[1] + uzp * vz[2];
   double r = p.dt * sum_y;
   r += p.dt * sum_z;
   r += u;
-  r += p.dt * source(p.P, g, p.e, out, ix, iy, iz);
+  r += p.dt * source_with(p.P, p.e, out, t);
   p.R[c] = r;
 }
 
@@ -392,6 +434,7 @@ template <int DIR>
 struct LineSource {
   double fin = 0.0, fout = 0.0;
   const double *zs = nullptr, *zc = nullptr;
+  MTS_HD LineSource() {}
   MTS_HD LineSource(const PhysConst &P, const DevGeom &g, int j, int k, const double *zt) {
     if (DIR == 2) {
       zs = zt ? zt : g.tab[2] + kTabSin * g.n[2];
@@ -938,15 +981,73 @@ MTS_HD void line_rows(const LineArgs &p, int j, int k, const LineMeta &lm, const
   }
 }
 
+// Right-hand side of rows [s, s + m) of a regular line (no MIB blocks, every
+// row on side `out`, whose coefficient is constant): the regular rows of
+// line_rows, operation for operation, without the matrix coefficients, which
+// the regular solve takes from its precomputed table.  The boundary rows'
+// couplings are not applied here.
+template <int DIR, int MMAX, int FLAGS, typename R = double>
+MTS_HD void regular_rhs(const LineArgs &p, int j, int k, bool out, const R *la, const R *lb,
+                        const double *wl, const LineSource<DIR> &src, const LaneSeg &sg,
+                        R (&d)[MMAX]) {
+  constexpr int RHS = FLAGS & 3;
+  const LinePart &q = p.q;
+  const int s = sg.s, m = sg.m;
+#pragma unroll
+  for (int r = 0; r < MMAX; r++) d[r] = 0.0;
+  if (m == 0) return;
+  const int base = 1 + sg.k * q.stride;
+  const int sl_prev = sg.k == 0 ? 0 : base - q.stride + q.m - 1;
+  const int sl_next = sg.k == q.p - 1 ? 1 + q.p * q.stride : base + q.stride;
+  (void)src;
+  const double *w = wl + (out ? 0 : 3);
+  const R w0 = (R)w[0], w1 = (R)w[1], w2 = (R)w[2];
+  const R rt = (R)p.theta_dt;
+  (void)rt;
+  (void)s;
+#pragma unroll
+  for (int r = 0; r < MMAX; r++) {
+    if (r < m) {
+      if (RHS == kRhsCopy) {
+        d[r] = la[base + r];
+      } else {
+        const R *f = RHS == kRhsDouglas ? lb : la;
+        const R fp = r == 0 ? f[sl_prev] : f[base + r - 1];
+        const R fc = f[base + r];
+        const R fn = r + 1 < m ? f[base + r + 1] : f[sl_next];
+        const R sum = fmad(fn, w2, fmad(fc, w1, fp * w0));
+        if (RHS == kRhsDouglas) {
+          d[r] = fmad(-(R)p.dt, sum, la[base + r]);
+        } else {
+          R v = fmad(rt, sum, fc);
+          if (RHS == kRhsTSZ) {
+            v = v + (R)(p.c_f1 * src.at(p.P, p.g, p.e_f1, out, j, k, s + r));
+            v = v + (R)(p.c_f2 * src.at(p.P, p.g, p.e_f2, out, j, k, s + r));
+          }
+          d[r] = v;
+        }
+      }
+    }
+  }
+}
+
+// Whether line lid is regular for the solve: no MIB blocks, and a constant
+// diffusion coefficient on its side (outside always; inside when beta is
+// constant).  Lines outside the interior (valid false) count as regular:
+// they solve nothing.
+MTS_HD bool regular_line(const PhysConst &P, const LineMeta &lm, bool valid) {
+  return !valid || (lm.bb == lm.be && (lm.out0 || P.beta_kind == 0));
+}
+
 // Epilogue of the lane's rows into the LDS output tile; the group's lane 0
 // also writes the two boundary rows (x0, xn: their values).
 template <int DIR, int MMAX, int FLAGS, typename R = double>
 MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, const LaneSeg &sg,
-                       const R (&x)[MMAX], double x0, double xn, R *lo, const double *zt) {
+                       const R (&x)[MMAX], double x0, double xn, R *lo,
+                       const LineSource<DIR> &src) {
   constexpr bool EPI = (FLAGS & kFlagEpiF) != 0;
   using M = LineMap<DIR>;
   const LinePart &q = p.q;
-  const LineSource<DIR> src(p.P, p.g, j, k, zt);
   (void)src;
   if (sg.m > 0) {
     const unsigned cm = EPI ? rows_outside(p.mib, lm, sg.s) : 0u;
@@ -972,6 +1073,13 @@ MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, cons
     lo[0] = (R)x0;
     lo[1 + q.p * q.stride] = (R)xn;
   }
+}
+
+template <int DIR, int MMAX, int FLAGS, typename R = double>
+MTS_HD void line_write(const LineArgs &p, int j, int k, const LineMeta &lm, const LaneSeg &sg,
+                       const R (&x)[MMAX], double x0, double xn, R *lo, const double *zt) {
+  const LineSource<DIR> src(p.P, p.g, j, k, zt);
+  line_write<DIR, MMAX, FLAGS, R>(p, j, k, lm, sg, x, x0, xn, lo, src);
 }
 
 // Boundary values of line (j, k) for the output time (rows 0 and n-1).

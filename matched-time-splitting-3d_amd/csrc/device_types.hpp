@@ -153,6 +153,22 @@ struct LineArgs {
   double e_jump;       // exp(-t) of the jumps
   double e_f1, c_f1;   // TS z: f(t^n) with c = dt/2;  LOD epilogue: f(t^{n+1}) with c = dt
   double e_f2, c_f2;   // TS z: f(t^{n+1}) with c = dt/2
+  // Regular lines' solve tables of the direction (regular_solve.hpp), in the
+  // field's type: reg[0] for the guarded solve, reg[1] for the unguarded one
+  // (kFlagFull); each holds the outside table, then (reg_sides 2: beta
+  // constant inside) the inside one, of reg_side elements each.
+  const void *reg[2];
+  int reg_side, reg_sides;
+  // Solve stages with split set: the stage's tiles split into regular ones
+  // (tiles[0], every line regular, at the stage's line split) and the others
+  // (tiles[1], at the split of gen_lanes lanes per line: fewer rows per lane,
+  // fewer registers, more workgroups per CU for the general solve), entries
+  // k * 65536 + bx (see tile_lists), ntiles[c] of each (a list with none is
+  // null).  split 0: one launch over every tile.
+  int split;
+  const int *tiles[2];
+  int ntiles[2];
+  int gen_lanes;
 };
 
 // Douglas stage 1 right-hand side: R = dt L_y u + dt L_z u + u + dt f  (regular

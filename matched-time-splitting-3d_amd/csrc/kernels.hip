@@ -155,10 +155,19 @@ __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int
 }
 
 // ------------------------------------------------ Douglas stage-1 RHS
-__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p) {
-  const int ix = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ix >= p.g.n[0]) return;
-  rhs1_node(p, ix, blockIdx.y, blockIdx.z);
+// One thread per node, nodes in x-fastest order, 256 per workgroup.  The
+// hardware deals workgroups to the 8 XCDs round-robin; workgroup b is given
+// node block (b mod 8) nper + b / 8, so each XCD walks one contiguous run of
+// z-planes and the y and z neighbours a block reads were fetched into that
+// XCD's L2 by the blocks just before it.
+constexpr int kXcds = 8;
+__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, unsigned nodes, unsigned nper) {
+  const unsigned b = (blockIdx.x % kXcds) * nper + blockIdx.x / kXcds;
+  const unsigned t = b * 256u + threadIdx.x;
+  if (t >= nodes) return;
+  const unsigned n0 = (unsigned)p.g.n[0], n1 = (unsigned)p.g.n[1];
+  const unsigned r = t / n0;
+  rhs1_node(p, (int)(t - r * n0), (int)(r % n1), (int)(r / n1));
 }
 
 __global__ __launch_bounds__(256) void spec_kernel(SpecArgs p) {
@@ -298,22 +307,37 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
 
 // --------------------------------------------------------------- launchers
 hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipStream_t s) {
-  LineArgs a = args;
-  if (!make_line_part(a.g.n[dir], a.q, a.min_lanes)) return hipErrorInvalidValue;
-  if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
-  const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
+  auto one = [&](const LineArgs &in, int min_lanes) -> hipError_t {
+    LineArgs a = in;
+    if (!make_line_part(a.g.n[dir], a.q, min_lanes)) return hipErrorInvalidValue;
+    if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
+    const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
 #define MTS_DL(D, LL) \
   if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, f32, a, s);
-  MTS_DL(0, 16) MTS_DL(0, 32) MTS_DL(0, 64)
-  MTS_DL(1, 16) MTS_DL(1, 32) MTS_DL(1, 64)
-  MTS_DL(2, 16) MTS_DL(2, 32) MTS_DL(2, 64)
+    MTS_DL(0, 16) MTS_DL(0, 32) MTS_DL(0, 64)
+    MTS_DL(1, 16) MTS_DL(1, 32) MTS_DL(1, 64)
+    MTS_DL(2, 16) MTS_DL(2, 32) MTS_DL(2, 64)
 #undef MTS_DL
-  return hipErrorInvalidValue;
+    return hipErrorInvalidValue;
+  };
+  if ((flags & kFlagSolve) && args.split) {
+    // regular tiles at the stage's split, general ones at gen_lanes lanes
+    LineArgs ar = args, ag = args;
+    ar.ntiles[1] = 0;
+    ag.ntiles[0] = 0;
+    hipError_t e = hipSuccess;
+    if (ag.ntiles[1] > 0) e = one(ag, args.gen_lanes > args.min_lanes ? args.gen_lanes : args.min_lanes);
+    if (e == hipSuccess && ar.ntiles[0] > 0) e = one(ar, args.min_lanes);
+    return e;
+  }
+  return one(args, args.min_lanes);
 }
 
 hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s) {
-  dim3 grid((a.g.n[0] + 255) / 256, a.g.n[1], a.g.n[2]);
-  hipLaunchKernelGGL(rhs1_kernel, grid, dim3(256), 0, s, a);
+  const size_t nodes = (size_t)a.g.n[0] * a.g.n[1] * a.g.n[2];
+  if (nodes == 0 || nodes > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
+  const unsigned nper = (unsigned)(((nodes + 255) / 256 + kXcds - 1) / kXcds);
+  hipLaunchKernelGGL(rhs1_kernel, dim3(nper * kXcds), dim3(256), 0, s, a, (unsigned)nodes, nper);
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@
 
 #include "device_core.hpp"
 #include "kernels.hpp"
+#include "regular_solve.hpp"
 
 #ifndef MTS_LINE_DIR
 #error "compile with -DMTS_LINE_DIR=0|1|2 -DMTS_LINE_L=16|32|64"
@@ -300,10 +301,75 @@ struct Pair2<float> {
   using type = float2;
 };
 
+// LDS layout of a line kernel: the field tile(s), the per-line weights, the z
+// axis table (z stages with a source term), then the regular lines' solve
+// tables (both sides, solve stages), 16-byte aligned.
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+struct LdsLayout {
+  static constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  static constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !SOLVE;
+  static constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || (FLAGS & 3) == kRhsTSZ);
+  static constexpr int T = tile_lines(L);
+  MTS_RS_HD static size_t wt_off(int lds_len) {
+    return (size_t)T * lds_len * sizeof(R) * (TWO ? 2 : 1);
+  }
+  MTS_RS_HD static size_t zt_off(int lds_len) { return wt_off(lds_len) + (size_t)T * kWtLen * sizeof(double); }
+  MTS_RS_HD static size_t reg_off(int lds_len, int n2) {
+    const size_t e = zt_off(lds_len) + (ZT ? 2 * (size_t)n2 * sizeof(double) : 0);
+    return (e + 15) / 16 * 16;
+  }
+  MTS_RS_HD static size_t bytes(int lds_len, int n2, bool reg) {
+    return reg_off(lds_len, n2) + (reg && SOLVE ? 2 * (size_t)reg_side_len(L, MMAX) * sizeof(R) : 0);
+  }
+  // the regular-tile kernel: tile(s), weights, then the solve tables of the
+  // sides present (reg_side elements each, two sides), no z table
+  MTS_RS_HD static size_t reg_bytes(int lds_len, int reg_side, int reg_sides) {
+    const size_t e = (zt_off(lds_len) + 15) / 16 * 16;
+    return e + ((size_t)reg_sides * reg_side * sizeof(R) + 15) / 16 * 16;
+  }
+};
+
 // R: the field type, double (FP64 form) or float (FP32 form: fields, LDS
 // tiles and the line solve in FP32, rows formed in FP64).
+// Tile (bx, by) of workgroup blockIdx.x (tile lists, see tile_lists) or of
+// (blockIdx.x, blockIdx.y) (the whole grid).  y/z tiles narrower than a
+// 128-byte line (T * sizeof(R) < 128): the G tiles of each line go to one XCD
+// (workgroups go to the 8 XCDs in turn), dispatched one after the other, so
+// the line is read from HBM once and the other parts come from that XCD's
+// L2.  (Otherwise neighbouring tiles stay on different XCDs at the same time:
+// giving each XCD a contiguous range of tiles instead is slower in every
+// direction.)
+template <int DIR, int T, typename R>
+__device__ __forceinline__ void tile_coords(const int *list, int &bx, int &by) {
+  constexpr unsigned G = tile_group<DIR, T, R>();
+  const unsigned nb = list ? gridDim.x : gridDim.x * gridDim.y;
+  const unsigned b = list ? blockIdx.x : blockIdx.y * gridDim.x + blockIdx.x;
+  unsigned v = b;
+  if (G > 1 && b < nb / (8 * G) * (8 * G)) v = ((b >> 3) / G * 8 + (b & 7)) * G + (b >> 3) % G;
+  if (list) {
+    const int e = list[v];
+    bx = tile_bx(e);
+    by = tile_k(e) - 1;
+  } else {
+    bx = (int)(v % gridDim.x);
+    by = (int)(v / gridDim.x);
+  }
+}
+
+// The tile walk of thread tid, formed from inputs the compiler must treat as
+// new: the per-pair addresses and LDS slots of one phase (load, LDS store,
+// write-back) are then recomputed there (a few integer operations) instead
+// of being kept in registers through the solve.
+template <int DIR, int T>
+__device__ __forceinline__ TileWalk<DIR, T> fresh_walk(const LineArgs &p, int j0, int k, int tid) {
+  asm volatile("" : "+v"(j0), "+v"(k), "+v"(tid));
+  return TileWalk<DIR, T>(p, j0, k, tid);
+}
+
+// One tile of one line stage: every row built from the tables (regular rows
+// and MIB blocks), the general partitioned solve.
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
-__global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
+__device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
   using V2 = typename Pair2<R>::type;
   constexpr int RHS = FLAGS & 3;
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
@@ -317,26 +383,9 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   const LinePart &q = p.q;
   const int n = g.n[DIR];
   const int nj = LineMap<DIR>::nj(g);
-  int bx = blockIdx.x, by = blockIdx.y;
-  constexpr unsigned G = tile_group<DIR, T, R>();
-  if (G > 1) {
-    // y/z tiles narrower than a 128-byte line (T * sizeof(R) < 128): place
-    // the G tiles of each line on one XCD (workgroups go to the 8 XCDs in
-    // turn), dispatched one after the other, so the line is read from HBM
-    // once and the other parts come from that XCD's L2.  (Otherwise
-    // neighbouring tiles stay on different XCDs at the same time: giving
-    // each XCD a contiguous range of tiles instead is slower in every
-    // direction.)
-    const unsigned nb = gridDim.x * gridDim.y;
-    const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
-    unsigned v = b;
-    if (b < nb / (8 * G) * (8 * G)) v = ((b >> 3) / G * 8 + (b & 7)) * G + (b >> 3) % G;
-    bx = v % gridDim.x;
-    by = v / gridDim.x;
-  }
   const int j0 = bx * T;
   const int k = by + 1;
-  if (j0 >= nj) return;  // the launch rounds the grid width up to a multiple of G
+  if (j0 >= nj) return;  // the whole-grid launch rounds its width up to a multiple of G
 
   extern __shared__ double smem[];
   R *tA = reinterpret_cast<R *>(smem);
@@ -395,9 +444,10 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
         if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].gi);
       }
     }
+    const TileWalk<DIR, T> walk_st = fresh_walk<DIR, T>(p, j0, k, tid);
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      const WalkPair &t = tp[c];
+      const WalkPair t = walk_st.at(p, u0 + c);
       if (t.ld0) {
         tA[t.sl0] = va[c].x;
         if (HAS_S) tB[t.sl0] = vb[c].x;
@@ -451,13 +501,14 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
   __syncthreads();
 
   const R *tO = TWO && !HAS_S ? tB : tA;
+  const TileWalk<DIR, T> walk_wb = fresh_walk<DIR, T>(p, j0, k, tid);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
     V2 vo[kChunk];
     WalkPair tp[kChunk];
 #pragma unroll
     for (int c = 0; c < kChunk; c++) {
-      tp[c] = walk.at(p, u0 + c);
+      tp[c] = walk_wb.at(p, u0 + c);
       vo[c].x = tp[c].w0 ? tO[tp[c].sl0] : R(0);
       vo[c].y = tp[c].w1 ? tO[tp[c].sl1] : R(0);
     }
@@ -475,20 +526,226 @@ __global__ __launch_bounds__(kBlock) void line_kernel(LineArgs p) {
 }
 
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void line_kernel(
+    LineArgs p) {
+  int bx, by;
+  tile_coords<DIR, tile_lines(L), R>(p.split ? p.tiles[1] : nullptr, bx, by);
+  line_body<DIR, L, MMAX, FLAGS, R>(p, bx, by);
+}
+
+// What one lane needs of one tile besides the field: the tile's place, the
+// lane's line (interior or not, side), its boundary values and the line's
+// source factors, the thread's entry of the tile's weight table and, for
+// in_analytic stages, of its end rows.  Gathered for the next tile while the
+// current one is solved.
+template <int DIR>
+struct RegTileMeta {
+  int j0, k;
+  bool valid, out0;
+  double x0, xn, wt_v, end_v;
+  int end_sl;
+  LineSource<DIR> src;
+};
+
+template <int DIR, int L>
+__device__ __forceinline__ RegTileMeta<DIR> reg_tile_meta(const LineArgs &p, int e,
+                                                          const double *zt) {
+  constexpr int T = tile_lines(L);
+  constexpr int LPW = 64 / L;
+  const int tid = threadIdx.x;
+  const int l = (tid >> 6) * LPW + (tid & 63) / L;
+  const int nj = LineMap<DIR>::nj(p.g);
+  RegTileMeta<DIR> m{};
+  m.j0 = tile_bx(e) * T;
+  m.k = tile_k(e);
+  const int j = m.j0 + l;
+  m.valid = j >= 1 && j <= nj - 2;
+  m.out0 = m.valid && p.mib.cls0[m.k * nj + j] != 0;
+  m.x0 = m.xn = 0.0;
+  if (m.valid) line_ends<DIR>(p, j, m.k, m.x0, m.xn);
+  m.wt_v = tid < T * kWtLen ? line_weight_value<DIR>(p, m.j0, m.k, tid) : 0.0;
+  m.end_sl = p.in_analytic && tid < 2 * T ? tile_end_slot<DIR>(p, m.j0, tid) : -1;
+  m.end_v = m.end_sl >= 0 ? tile_end_val<DIR>(p, m.j0, m.k, tid) : 0.0;
+  if (m.valid) m.src = LineSource<DIR>(p.P, p.g, j, m.k, zt);
+  return m;
+}
+
+// Solve stage, regular tiles (every line regular: no MIB block, a constant
+// coefficient on its side): the right-hand-side-only solve with the
+// precomputed tables, one tile per workgroup.  None of the general row, block
+// and solve code is here, so the kernel needs far fewer registers and LDS
+// than line_kernel and four of its workgroups share a CU: while some load
+// their tile, others solve or store theirs.
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void line_kernel_reg(LineArgs p) {
+  using V2 = typename Pair2<R>::type;
+  constexpr int RHS = FLAGS & 3;
+  constexpr bool HAS_S = (RHS == kRhsDouglas);
+  constexpr bool FULL = (FLAGS & kFlagFull) != 0;
+  constexpr int T = tile_lines(L);
+  constexpr int kChunk = kTileChunk;
+  using LL = LdsLayout<DIR, L, MMAX, FLAGS, R>;
+  const DevGeom &g = p.g;
+  const LinePart &q = p.q;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int kseg = lane & (L - 1);
+  const int l = (tid >> 6) * (64 / L) + lane / L;
+
+  extern __shared__ double smem[];
+  R *tA = reinterpret_cast<R *>(smem);
+  R *tB = tA + T * q.lds_len;
+  double *wt = reinterpret_cast<double *>(reinterpret_cast<char *>(smem) + LL::wt_off(q.lds_len));
+  R *lreg = reinterpret_cast<R *>(reinterpret_cast<char *>(smem) + (LL::zt_off(q.lds_len) + 15) / 16 * 16);
+  const R *fA = static_cast<const R *>(p.A);
+  const R *fS = static_cast<const R *>(p.S);
+  R *fO = static_cast<R *>(p.out);
+
+  // this tile: the lane's line, boundary values and source factors (the z
+  // stages read sin z / cos z from the axis tables, not an LDS copy), the
+  // solve table of the sides present, then the field, all loads in flight
+  // together
+  const RegTileMeta<DIR> cur = reg_tile_meta<DIR, L>(p, p.tiles[0][blockIdx.x], nullptr);
+  {
+    // the solve tables straight into LDS (16-byte LDS-DMA loads: lane i of
+    // a wavefront writes 16 bytes at the instruction's LDS base + 16 i); the
+    // host pads them to whole 16-byte units
+    const char *greg = static_cast<const char *>(p.reg[FULL ? 1 : 0]);
+    const int units = (p.reg_sides * p.reg_side * (int)sizeof(R) + 15) / 16;
+    const int wave = tid >> 6;
+    for (int u0 = wave * 64; u0 < units; u0 += kBlock) {
+      if (u0 + lane < units)
+        __builtin_amdgcn_global_load_lds(
+            greg + (size_t)(u0 + lane) * 16,
+            (__attribute__((address_space(3))) void *)(reinterpret_cast<char *>(lreg) +
+                                                         (size_t)u0 * 16),
+            16, 0, 0);
+    }
+  }
+  const TileWalk<DIR, T> walk(p, cur.j0, cur.k, tid);
+  V2 va[kChunk], vb[kChunk];
+#pragma unroll
+  for (int c = 0; c < kChunk; c++) {
+    const WalkPair w = walk.at(p, c);
+    va[c].x = va[c].y = R(0);
+    vb[c].x = vb[c].y = R(0);
+    if (w.ld0 || w.ld1) {
+      va[c] = *reinterpret_cast<const V2 *>(fA + w.gi);
+      if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + w.gi);
+    }
+  }
+  const TileWalk<DIR, T> walk_st = fresh_walk<DIR, T>(p, cur.j0, cur.k, tid);
+#pragma unroll
+  for (int c = 0; c < kChunk; c++) {
+    const WalkPair w = walk_st.at(p, c);
+    if (w.ld0) {
+      tA[w.sl0] = va[c].x;
+      if (HAS_S) tB[w.sl0] = vb[c].x;
+    }
+    if (w.ld1) {
+      tA[w.sl1] = va[c].y;
+      if (HAS_S) tB[w.sl1] = vb[c].y;
+    }
+  }
+  if (tid < T * kWtLen) wt[tid] = cur.wt_v;
+  if (cur.end_sl >= 0) tA[cur.end_sl] = (R)cur.end_v;
+  __builtin_amdgcn_s_waitcnt(0);  // the LDS-DMA loads are in LDS
+  __syncthreads();
+
+  {
+    const int j = cur.j0 + l;
+    const LaneSeg sg = lane_seg(q, kseg, cur.valid);
+    R d[MMAX];
+    regular_rhs<DIR, MMAX, FLAGS>(p, j, cur.k, cur.out0, tA + l * q.lds_len, tB + l * q.lds_len,
+                                  wt + l * kWtLen, cur.src, sg, d);
+    const double *w = wt + l * kWtLen + (cur.out0 ? 0 : 3);
+    const R rt = (R)p.theta_dt;
+    const R a_first = -rt * (R)w[0], c_last = -rt * (R)w[2];
+    const bool first = sg.m > 0 && sg.k == 0;
+    d[0] = first ? (R)((double)d[0] - (double)a_first * cur.x0) : d[0];
+    const bool lastseg = sg.m > 0 && sg.k == q.p - 1;
+#pragma unroll
+    for (int r = 0; r < MMAX; r++) {
+      const bool hit = lastseg && r == sg.m - 1;
+      d[r] = hit ? (R)((double)d[r] - (double)c_last * cur.xn) : d[r];
+    }
+    const R *tb = lreg + (cur.out0 ? 0 : p.reg_side);
+    const R *rows = tb + reg_rows_off(kseg == q.p - 1 ? 1 : 0, MMAX);
+    const R *sc = tb + reg_scal_off(kseg, MMAX);
+    R D1, D2;
+    reg_forward<MMAX, FULL, R>(rows, sc, d, sg.m, D1, D2);
+    const R D1n = grp_down<L>(D1, 1, kseg, R(0));
+    R ed = reg_fold<R>(sc, FULL ? MMAX : sg.m, D1, D2, D1n);
+#pragma unroll
+    for (int lv = 0, s = 1; s < L; lv++, s <<= 1) {
+      const R dm = grp_up<L>(ed, s, kseg, R(0)), dp = grp_down<L>(ed, s, kseg, R(0));
+      ed = reg_level<R>(sc, lv, ed, dm, dp);
+    }
+    const R xe = ed * sc[kRegQ];
+    const R xprev = grp_up<L>(xe, 1, kseg, R(0));
+    reg_back<MMAX, FULL, R>(rows, sc, d, sg.m, D1, xe, xprev);
+    if (cur.valid) {
+      LineMeta lm;
+      lm.bb = lm.be = 0;
+      lm.out0 = cur.out0;
+      lm.seq = false;
+      line_write<DIR, MMAX, FLAGS>(p, j, cur.k, lm, sg, d, cur.x0, cur.xn, tA + l * q.lds_len,
+                                   cur.src);
+    }
+  }
+  __syncthreads();
+
+  const TileWalk<DIR, T> walk_wb = fresh_walk<DIR, T>(p, cur.j0, cur.k, tid);
+#pragma unroll
+  for (int c = 0; c < kChunk; c++) {
+    const WalkPair w = walk_wb.at(p, c);
+    V2 vo;
+    vo.x = w.w0 ? tA[w.sl0] : R(0);
+    vo.y = w.w1 ? tA[w.sl1] : R(0);
+    if (w.w0 && w.w1) {
+      *reinterpret_cast<V2 *>(fO + w.gi) = vo;
+    } else {
+      if (w.w0) fO[w.gi] = vo.x;
+      if (w.w1) fO[w.gi + 1] = vo.y;
+    }
+  }
+}
+
+// A solve stage with tile lists runs the regular tiles with line_kernel_reg
+// and the others with line_kernel; any other stage runs line_kernel over the
+// whole grid.
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
 hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   const DevGeom &g = a.g;
   const int nj = DIR == 0 ? g.n[1] : g.n[0];
   const int nk = DIR == 2 ? g.n[1] : g.n[2];
-  constexpr bool TWO = ((FLAGS & 3) == kRhsDouglas) || !(FLAGS & kFlagSolve);
   constexpr int T = tile_lines(L);
-  constexpr bool ZT = DIR == 2 && ((FLAGS & kFlagEpiF) || (FLAGS & 3) == kRhsTSZ);
-  const size_t shm = (size_t)T * a.q.lds_len * sizeof(R) * (TWO ? 2 : 1) +
-                    (size_t)T * kWtLen * sizeof(double) + (ZT ? 2 * (size_t)g.n[2] * sizeof(double) : 0);
+  constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
+  constexpr bool FULL = (FLAGS & kFlagFull) != 0;
+  using LL = LdsLayout<DIR, L, MMAX, FLAGS, R>;
+  if (SOLVE && a.split) {
+    if (a.ntiles[1] > 0) {
+      hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), dim3(a.ntiles[1]), dim3(kBlock),
+                         LL::bytes(a.q.lds_len, g.n[2], false), st, a);
+      const hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    if (a.ntiles[0] > 0) {
+      if (!a.reg[FULL ? 1 : 0]) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((line_kernel_reg<DIR, L, MMAX, FLAGS, R>), dim3(a.ntiles[0]), dim3(kBlock),
+                         LL::reg_bytes(a.q.lds_len, a.reg_side, a.reg_sides), st, a);
+      return hipGetLastError();
+    }
+    return hipSuccess;
+  }
+  LineArgs b = a;
+  b.split = 0;
+  const size_t shm = LL::bytes(a.q.lds_len, g.n[2], false);
   int gx = (nj + T - 1) / T;
   constexpr int G = (int)tile_group<DIR, T, R>();
-  gx = (gx + G - 1) / G * G;  // tiles in groups of G (see line_kernel)
+  gx = (gx + G - 1) / G * G;  // tiles in groups of G (see tile_coords)
   dim3 grid(gx, nk - 2);
-  hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), grid, dim3(kBlock), shm, st, a);
+  hipLaunchKernelGGL((line_kernel<DIR, L, MMAX, FLAGS, R>), grid, dim3(kBlock), shm, st, b);
   return hipGetLastError();
 }
 

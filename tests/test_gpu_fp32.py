@@ -87,8 +87,13 @@ def test_fp32_footprint_and_serving(mts, ctx):
     i64, i32, _ = _pair(mts, ctx, case)
     b64, b32 = i64.device_bytes(), i32.device_bytes()
     # x rows padded to 128 bytes: pitch 80 doubles or 96 floats for 65 nodes;
-    # field plus the node classification byte, the rest the same
-    assert b64 - 80 * 65 * 65 * 9 == b32 - 96 * 65 * 65 * 5
+    # field plus the node classification byte; the regular lines' solve
+    # tables in the field's type (65-node lines: 16-lane groups of 4-row
+    # segments, guarded and unguarded tables, the outside side only (beta
+    # varies inside), three directions: 3 * 2 * (2 * 4 * 6 + 16 * 20)
+    # elements, whole 16-byte units either way); the rest the same
+    tables = 3 * 2 * (2 * 4 * 6 + 16 * 20)
+    assert b64 - 80 * 65 * 65 * 9 - 8 * tables == b32 - 96 * 65 * 65 * 5 - 4 * tables
     i32.initialize(0.0)
     steps, t_end = i32.run(0.0, case[6] * case[7])
     alone = i32.error(t_end)
