@@ -209,6 +209,17 @@ int mts_sync(mts_instance *inst);
  * (Error, douglas_adi.cc:1723-1770). */
 int mts_error_norms(mts_instance *inst, double t, double *linf, double *l2, int argmax[3]);
 
+/* mts_error_norms without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  The per-block partials and their
+ * reduction live in a fixed device array of the library (48 KiB per GPU, held
+ * until the process ends; at most 2048 partials, which covers every grid),
+ * not in the instance, and the reduction runs on the device.  Same results as
+ * mts_error_norms, bit for bit.  All calls on one GPU share that array: calls
+ * that overlap on a GPU (other host threads, other instances' streams) race. */
+int mts_error_norms_noalloc(mts_instance *inst, double t, double *linf, double *l2,
+                            int argmax[3]);
+
 /* Device pointer of the solution field (x pitch in *pitch, in elements) and
  * the instance's HIP stream, for callers that keep data on the device.  For an
  * MTS_FP32 instance the pointer addresses floats. */

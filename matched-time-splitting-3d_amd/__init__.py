@@ -135,6 +135,7 @@ def lib() -> C.CDLL:
             "mts_run": ([VP, D, D, P(I), P(D)], I),
             "mts_sync": ([VP], I),
             "mts_error_norms": ([VP, D, P(D), P(D), P(I)], I),
+            "mts_error_norms_noalloc": ([VP, D, P(D), P(D), P(I)], I),
             "mts_field_device": ([VP, P(VP), P(I), P(VP)], I),
             "mts_tdma_batched": ([VP, I, I, VP, VP, VP, VP, VP], I),
             "mts_profile_enable": ([VP, I], I),

@@ -134,6 +134,7 @@ struct mts_lu {
 struct mts_ctx {
   int device = 0;
   hipDeviceProp_t prop;
+  void *norms_scratch = nullptr;  // device address of the library's fixed norms array
 };
 
 struct mts_instance {
@@ -737,6 +738,7 @@ int mts_ctx_create(int device, mts_ctx **out) {
   c->device = device;
   MTS_HIP(hipSetDevice(device));
   MTS_HIP(hipGetDeviceProperties(&c->prop, device));
+  MTS_HIP(mts::norms_scratch_ptr(device, &c->norms_scratch));
   *out = c.release();
   return MTS_OK;
 }
@@ -1090,6 +1092,31 @@ int mts_error_norms(mts_instance *in, double t, double *linf, double *l2, int ar
     argmax[0] = (int)(mi / (ny * nz));
     argmax[1] = (int)((mi / nz) % ny);
     argmax[2] = (int)(mi % nz);
+  }
+  return MTS_OK;
+}
+
+int mts_error_norms_noalloc(mts_instance *in, double t, double *linf, double *l2,
+                            int argmax[3]) {
+  if (!in) return fail(MTS_E_ARG, "null instance");
+  if (!in->ctx->norms_scratch) return fail(MTS_E_DEVICE, "the norms array has no device address");
+  if (mts::norms_blocks(in->geom) > mts::kNormsMaxParts)
+    return fail(MTS_E_ARG, "grid needs more norm partials than the fixed array holds");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  mts::NormsResult *dres = nullptr;
+  MTS_HIP(mts::launch_norms_scratch(in->geom, in->field(), in->f32(), std::exp(-t),
+                                    in->ctx->norms_scratch, in->stream, &dres));
+  mts::NormsResult res;
+  MTS_HIP(hipMemcpyAsync(&res, dres, sizeof res, hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipStreamSynchronize(in->stream));
+  const mts::Grid &g = in->grid;
+  const long long ny = g.n[1], nz = g.n[2];
+  if (linf) *linf = res.linf;
+  if (l2) *l2 = std::sqrt(res.sumsq / (1.0 * g.n[0] * g.n[1] * g.n[2]));
+  if (argmax) {
+    argmax[0] = (int)(res.argmax / (ny * nz));
+    argmax[1] = (int)((res.argmax / nz) % ny);
+    argmax[2] = (int)(res.argmax % nz);
   }
   return MTS_OK;
 }

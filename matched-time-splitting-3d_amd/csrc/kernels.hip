@@ -23,6 +23,9 @@
 // disabled for this file) except the tridiagonal elimination order.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <vector>
+
 #include "device_core.hpp"
 #include "kernels.hpp"
 
@@ -270,6 +273,28 @@ __global__ __launch_bounds__(256) void norms_kernel(DevGeom g, const R *u, doubl
   }
 }
 
+// The partials reduced in the order of the host loop of mts_error_norms
+// (first maximum by reference index, squares summed part 0 first), so the
+// result is bitwise the same.  One thread: at most kNormsMaxParts partials.
+__global__ __launch_bounds__(64) void norms_final_kernel(const double *pmax,
+                                                         const long long *pidx,
+                                                         const double *psq, int nparts,
+                                                         NormsResult *out) {
+  if (threadIdx.x != 0) return;
+  double mx = -1.0, sq = 0.0;
+  long long mi = 0;
+  for (int i = 0; i < nparts; i++) {
+    if (pmax[i] > mx || (pmax[i] == mx && pidx[i] < mi)) {
+      mx = pmax[i];
+      mi = pidx[i];
+    }
+    sq += psq[i];
+  }
+  out->linf = mx;
+  out->sumsq = sq;
+  out->argmax = mi;
+}
+
 // ----------------------------------------------------- batched tridiagonal
 template <int MMAX>
 __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const double *A,
@@ -303,7 +328,69 @@ __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const doub
     if (r < m) U[base + s + r] = solved_row(d, r, m, xs, xe);
 }
 
+// Device memory of the allocation-free error norms (mts_error_norms_noalloc):
+// the per-block partials, then the reduced result.  Each GPU holds its own
+// copy from the library's load until the process ends.  In host code the name
+// is only a handle for hipGetSymbolAddress; kernels get the address it
+// returns (norms_scratch_ptr).
+constexpr size_t kScratchPmax = 0;
+constexpr size_t kScratchPsq = kScratchPmax + kNormsMaxParts * sizeof(double);
+constexpr size_t kScratchPidx = kScratchPsq + kNormsMaxParts * sizeof(double);
+constexpr size_t kScratchResult = kScratchPidx + kNormsMaxParts * sizeof(long long);
+constexpr size_t kScratchBytes = kScratchResult + 256;
+static_assert(kScratchBytes == kNormsScratchBytes, "scratch layout");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
 }  // namespace
+
+// ------------------------------------------------------ allocation-free norms
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch_ptr;  // per device, filled once
+}  // namespace
+
+hipError_t norms_scratch_ptr(int device, void **ptr) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (device < 0) return hipErrorInvalidDevice;
+  if ((size_t)device < g_scratch_ptr.size() && g_scratch_ptr[(size_t)device]) {
+    *ptr = g_scratch_ptr[(size_t)device];
+    return hipSuccess;
+  }
+  int cur = -1;
+  hipError_t e = hipGetDevice(&cur);
+  if (e != hipSuccess) return e;
+  if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+  void *p = nullptr;
+  e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  if (g_scratch_ptr.size() <= (size_t)device) g_scratch_ptr.resize((size_t)device + 1, nullptr);
+  g_scratch_ptr[(size_t)device] = p;
+  *ptr = p;
+  return hipSuccess;
+}
+
+hipError_t launch_norms_scratch(const DevGeom &g, const void *u, bool f32, double e,
+                                void *scratch_ptr, hipStream_t s, NormsResult **result) {
+  if (!scratch_ptr) return hipErrorInvalidValue;
+  const int nb = norms_blocks(g);
+  if (nb < 1 || nb > kNormsMaxParts) return hipErrorInvalidValue;
+  unsigned char *base = static_cast<unsigned char *>(scratch_ptr);
+  double *pmax = reinterpret_cast<double *>(base + kScratchPmax);
+  double *psq = reinterpret_cast<double *>(base + kScratchPsq);
+  long long *pidx = reinterpret_cast<long long *>(base + kScratchPidx);
+  NormsResult *res = reinterpret_cast<NormsResult *>(base + kScratchResult);
+  if (f32)
+    hipLaunchKernelGGL(norms_kernel<float>, dim3(nb), dim3(256), 0, s, g,
+                       static_cast<const float *>(u), e, pmax, pidx, psq);
+  else
+    hipLaunchKernelGGL(norms_kernel<double>, dim3(nb), dim3(256), 0, s, g,
+                       static_cast<const double *>(u), e, pmax, pidx, psq);
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(norms_final_kernel, dim3(1), dim3(64), 0, s, pmax, pidx, psq, nb, res);
+  *result = res;
+  return hipGetLastError();
+}
 
 // --------------------------------------------------------------- launchers
 hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipStream_t s) {
@@ -373,7 +460,7 @@ hipError_t launch_init(const DevGeom &g, void *u, bool f32, double e, hipStream_
 int norms_blocks(const DevGeom &g) {
   long long total = (long long)g.n[0] * g.n[1] * g.n[2];
   long long b = (total + 255) / 256;
-  return (int)(b < 2048 ? b : 2048);
+  return (int)(b < kNormsMaxParts ? b : kNormsMaxParts);
 }
 
 hipError_t launch_norms(const DevGeom &g, const void *u, bool f32, double e, double *part_max,

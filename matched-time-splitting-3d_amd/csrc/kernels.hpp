@@ -43,6 +43,22 @@ hipError_t launch_norms(const DevGeom &g, const void *u, bool f32, double e, dou
                         long long *part_idx, double *part_sq, int *nparts, hipStream_t s);
 int norms_blocks(const DevGeom &g);
 
+// Error norms with the partials and their reduction in a fixed device array
+// of the library (kNormsScratchBytes per GPU): no device allocation.  The
+// reduced record is left in that array at *result (a device address).  Calls
+// on one GPU share the array: calls that overlap on a GPU race.
+constexpr int kNormsMaxParts = 2048;  // partials of one norms launch, at most
+constexpr size_t kNormsScratchBytes = (size_t)kNormsMaxParts * 24 + 256;
+struct NormsResult {
+  double linf, sumsq;
+  long long argmax;  // reference-order node index of the first maximum
+};
+// The array's device address on device (once per device; the device must be
+// current on the first call).
+hipError_t norms_scratch_ptr(int device, void **ptr);
+hipError_t launch_norms_scratch(const DevGeom &g, const void *u, bool f32, double e,
+                                void *scratch_ptr, hipStream_t s, NormsResult **result);
+
 // Batched tridiagonal solve: nlines independent systems of length n, arrays
 // line-major (line l, row i at l*n + i).  a[l*n] and c[l*n + n-1] are ignored.
 hipError_t launch_tdma(int n, int nlines, const double *a, const double *b, const double *c,
