@@ -191,7 +191,7 @@ def main():
 
     cpu = None
     if args.cpu_baseline:
-        cpu = cpu_baseline(mts, st, args.method, dt, nodes, phys)
+        cpu = cpu_baseline(mts, st, args, cf, dt, nodes, phys)
 
     line = {
         "metric": "grid-pt-steps/s at fixed L-inf error (FP32+batch vs FP64); solves/s + p50/p99",
@@ -538,9 +538,36 @@ def copy_calibration():
     return 0 if ok else 1
 
 
-def cpu_baseline(mts, st, method, dt, nodes, phys):
-    """The checker restatement (plain C, one thread) on a bounded sample of the
-    same workload: whole steps of the same run until ~10 s have passed."""
+REF_BIN = os.path.join(ROOT, "oracle", "_ref", "mts_ref")
+
+
+def reference_run(cf, args, n_intervals, dt, steps):
+    """The reference solver compiled from its own sources (oracle/_ref/mts_ref)
+    for `steps` steps of the workload at n_intervals; (step-loop seconds, steps)
+    or None when it is not built or stops on its own."""
+    import subprocess
+    import tempfile
+    if not os.path.exists(REF_BIN):
+        return None
+    kind, *p = cf["surf"]
+    bk, b0, b1 = cf["beta"]
+    with tempfile.TemporaryDirectory() as out:
+        cmd = [REF_BIN, "--method", args.method, "--mib", str(args.mib), "--n", str(n_intervals),
+               "--lo", "-1", "--hi", "1", "--dt", repr(dt), "--T", repr(cf["T"]),
+               "--surface", f"{kind}:" + ",".join(repr(float(v)) for v in p),
+               "--beta", f"{bk}:{float(b0)!r},{float(b1)!r}", "--out", out,
+               "--max-steps", str(steps), "--no-dump"]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
+        path = os.path.join(out, "summary.txt")
+        if r.returncode != 0 or not os.path.exists(path):
+            return None
+        kv = dict(l.split(None, 1) for l in open(path).read().splitlines() if l.strip())
+    return float(kv["step_loop_s"]), int(kv["steps"])
+
+
+def port_run(mts, st, method, dt, phys, budget_s, max_steps=4096):
+    """The checker restatement (plain C, one thread): whole steps of the run
+    until budget_s has passed; (seconds, steps)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import mts_oracle
 
@@ -550,20 +577,63 @@ def cpu_baseline(mts, st, method, dt, nodes, phys):
     t = 0.0
     steps = 0
     spent = 0.0
-    while steps < 1 or (spent < 10.0 and steps < 4096):
+    while steps < 1 or (spent < budget_s and steps < max_steps):
         t0 = time.perf_counter()
         u, _ = op.run(method, dt, t, 1, u)
         spent += time.perf_counter() - t0
         t += dt
         steps += 1
-    return {
-        "value": nodes * steps / spent,
+    return spent, steps
+
+
+def cpu_baseline(mts, st, args, cf, dt, nodes, phys):
+    """The reference's CPU path on the box's host, one core (it is
+    single-threaded): the reference solver itself, built from its own sources,
+    on a bounded sample of the same run (its first steps) where the reference
+    completes this workload; where it stops on its own (the exit(0) of its
+    tangential-jump set-up), the plain-C restatement on the same run, with the
+    reference/restatement speed ratio measured on the half-size grid of the
+    workload beside it."""
+    side = round(nodes ** (1 / 3))
+    if st.reference_stops()["first_kind"] < 0:
+        steps = 2 if nodes > 4e6 else max(2, min(64, int(3e7 / nodes)))
+        r = reference_run(cf, args, args.n, dt, steps)
+        if r is not None:
+            spent, k = r
+            return {
+                "value": nodes * k / spent,
+                "unit": "grid-pt-steps/s",
+                "cores": 1,
+                "kind": "reference",
+                "sample": f"the reference solver (built from its sources), the first {k} step(s) "
+                          f"of the same {side}^3 run, {spent:.2f} s on one host core "
+                          f"(set-up excluded)",
+            }
+    spent, k = port_run(mts, st, args.method, dt, phys, 10.0)
+    out = {
+        "value": nodes * k / spent,
         "unit": "grid-pt-steps/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"{steps} step(s) of the same {round(nodes ** (1 / 3))}^3 run, "
-                  f"{spent:.2f} s on one host core",
+        "sample": f"{k} step(s) of the same {side}^3 run, {spent:.2f} s on one host core "
+                  f"(the reference solver stops on its own at this size: "
+                  f"{st.reference_stops()['message']})",
     }
+    # reference / restatement speed on the half-size grid of the same workload
+    nh = args.n // 2
+    dth = 2.0 / nh
+    sh = mts.Setup(mts.grid(nh + 1), mts.surface(cf["surf"][0], *cf["surf"][1:]), phys,
+                   mib=args.mib)
+    if sh.reference_stops()["first_kind"] < 0:
+        r = reference_run(cf, args, nh, dth, 2)
+        if r is not None:
+            ps, pk = port_run(mts, sh, args.method, dth, phys, 1e9, max_steps=2)
+            ratio = (r[1] / r[0]) / (pk / ps)
+            out["reference_over_port"] = ratio
+            out["calibration"] = (f"{nh + 1}^3 grid of the same workload: reference {r[1]} steps "
+                                  f"in {r[0]:.2f} s, restatement {pk} steps in {ps:.2f} s; the "
+                                  f"reference-equivalent rate here is value x reference_over_port")
+    return out
 
 
 if __name__ == "__main__":

@@ -132,6 +132,28 @@ int mts_setup_diagnostics(const mts_setup *s, double t, double fp[2], double jum
 int mts_setup_crossing_errors(const mts_setup *s, int dir, double t, double *errl, double *errr,
                               double *jerr, size_t count);
 
+/* Where the reference itself would have stopped a run that this library
+ * completes.  Its set-up prepares numeric tangential flux-jump approximations
+ * for every crossing (Auxiliary_eta_* / Auxiliary_tau_*,
+ * spatial/intersection.cc:2746-3660) and every step evaluates them (Eta_* /
+ * Tau_*, :3670-4438); their results reach no output (approximate-jump mode,
+ * constant.h:16), so they are not computed here, but their geometric checks
+ * call exit(0).  kinds: 0 an auxiliary point outside the domain (set-up
+ * exit), 1 no three same-side nodes on both neighbouring lines (set-up exit),
+ * 2 an auxiliary point on a grid node or undefined (the reference leaves its
+ * node index unset: undefined behaviour, no exit), 3 a zero tangent-frame
+ * component (exit at the first step).  first_kind is the kind of the first
+ * exit in the reference's order (-1: it would run); first_dir / first_index
+ * name that crossing (line order, as mts_setup_crossings) and message holds
+ * the text the reference prints there. */
+typedef struct {
+  int count[4];       /* crossings per kind */
+  int first_kind;
+  int first_dir, first_index;
+  char message[160];
+} mts_reference_stops;
+int mts_setup_reference_stops(const mts_setup *s, mts_reference_stops *out);
+
 /* Node classification alone, the Mesh constructor (mesh/mesh.cc:20-70):
  * +1 outside, -1 inside, MTS_LAYOUT_XFAST; MTS_E_GEOMETRY when the interface
  * passes through a node (where the reference exits). */

@@ -74,6 +74,11 @@ class StageTime(C.Structure):
     _fields_ = [("name", C.c_char * 32), ("launches", C.c_int), ("total_ms", C.c_double)]
 
 
+class ReferenceStops(C.Structure):
+    _fields_ = [("count", C.c_int * 4), ("first_kind", C.c_int), ("first_dir", C.c_int),
+                ("first_index", C.c_int), ("message", C.c_char * 160)]
+
+
 class SetupInfo(C.Structure):
     _fields_ = [("n", C.c_int * 3), ("n_cross", C.c_longlong * 3),
                 ("corner_pairs", C.c_int * 3), ("singular_warnings", C.c_int), ("mib", C.c_int)]
@@ -118,6 +123,7 @@ def lib() -> C.CDLL:
             "mts_setup_create": ([P(GridDesc), P(SurfaceDesc), P(PhysicsDesc), I, I, P(VP)], I),
             "mts_setup_destroy": ([VP], None),
             "mts_setup_get_info": ([VP, P(SetupInfo)], I),
+            "mts_setup_reference_stops": ([VP, P(ReferenceStops)], I),
             "mts_setup_indicator": ([VP, VP, C.c_size_t], I),
             "mts_setup_coords": ([VP, I, VP, C.c_size_t], I),
             "mts_setup_crossings": ([VP, I, VP, C.c_size_t], I),
@@ -240,6 +246,14 @@ class Setup:
         i = SetupInfo()
         _check(lib().mts_setup_get_info(self._h, C.byref(i)))
         return i
+
+    def reference_stops(self) -> dict:
+        """Where the reference's own run would have stopped (exit(0) in its
+        tangential-jump set-up or first step); first_kind -1: it would run."""
+        r = ReferenceStops()
+        _check(lib().mts_setup_reference_stops(self._h, C.byref(r)))
+        return {"count": list(r.count), "first_kind": r.first_kind, "first_dir": r.first_dir,
+                "first_index": r.first_index, "message": r.message.decode()}
 
     @property
     def shape(self):

@@ -1,5 +1,6 @@
 // C ABI of the Matched-TS per-step path (include/mts.h).
 #include "mts.h"
+#include "ref_checks.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -148,6 +149,8 @@ struct mts_instance {
   size_t field_elems = 0;
   StreamHolder sh;                // the instance's stream (outlives the buffers below)
   hipStream_t stream = nullptr;   // == sh.s
+  StreamHolder side_sh;           // second stream of the split solve stages
+  mts::SideStream side;           // side_sh.s and its fork / join events (null: one stream)
   int precision = MTS_FP64;       // field storage and line solve
   DevBuf<double> u, r;            // solution (FP64 form); Douglas stage scratch
   DevBuf<float> uf;               // solution (FP32 form)
@@ -194,6 +197,9 @@ struct mts_instance {
   int g_steps = 0;
   ~mts_instance() {
     if (stream) (void)hipStreamSynchronize(stream);  // pending copies may read host staging
+    if (side.s) (void)hipStreamSynchronize(side.s);
+    if (side.fork) (void)hipEventDestroy(side.fork);
+    if (side.join) (void)hipEventDestroy(side.join);
     if (gexec) (void)hipGraphExecDestroy(gexec);
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
@@ -360,6 +366,11 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       const int v = std::atoi(ev);
       if (v == 16 || v == 32 || v == 64) p->min_lanes = v;
     }
+    // MTS_GEN_LANES=32|64: the same for the general tiles of the solve stages
+    if (const char *ev = std::getenv("MTS_GEN_LANES")) {
+      const int v = std::atoi(ev);
+      if (v == 16 || v == 32 || v == 64) p->gen_lanes = v;
+    }
     for (int d = 0; d < 3; d++) {
       mts::LinePart q;
       if (!mts::make_line_part(g.n[d], q, p->min_lanes))
@@ -378,6 +389,12 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       for (int iy = 0; iy < g.n[1]; iy++)
         std::memcpy(&p->ind[((size_t)iz * g.n[1] + iy) * p->px],
                     &setup->res.ind[g.idx(0, iy, iz)], g.n[0]);
+    // Douglas-ADI: the nodes spec_node writes carry +-2 (the sign is still
+    // the side), so the regular stage-1 stencil leaves them alone
+    for (size_t r = 0; r + mts::kSpecLen <= t.spec.size(); r += mts::kSpecLen) {
+      int8_t &v = p->ind[((size_t)t.spec[r + 2] * g.n[1] + t.spec[r + 1]) * p->px + t.spec[r]];
+      v = v > 0 ? 2 : -2;
+    }
     using namespace mts;
     std::vector<std::pair<int, int>> kinds;  // (direction, stage kind) this method launches
     for (int d = 0; d < 3; d++) {
@@ -463,6 +480,17 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   in->field_elems = p.field_elems;
   MTS_HIP(hipStreamCreateWithFlags(&in->sh.s, hipStreamNonBlocking));
   in->stream = in->sh.s;
+  // MTS_SIDE_STREAM=1: the general and the regular tiles of a solve stage run
+  // side by side on two streams (default: one after the other on the
+  // instance's stream; the fork and join cost more than the overlap gains
+  // outside graphs)
+  const char *ss = std::getenv("MTS_SIDE_STREAM");
+  if (ss && std::atoi(ss) == 1) {
+    MTS_HIP(hipStreamCreateWithFlags(&in->side_sh.s, hipStreamNonBlocking));
+    in->side.s = in->side_sh.s;
+    MTS_HIP(hipEventCreateWithFlags(&in->side.fork, hipEventDisableTiming));
+    MTS_HIP(hipEventCreateWithFlags(&in->side.join, hipEventDisableTiming));
+  }
 
   const mts::DeviceTableHost &t = p.t;
   in->phys = t.phys;
@@ -536,6 +564,19 @@ extern "C" {
 
 const char *mts_last_error(void) { return g_err.c_str(); }
 const char *mts_version(void) { return "mts-amd 0.1 (gfx950)"; }
+
+int mts_setup_reference_stops(const mts_setup *s, mts_reference_stops *out) {
+  if (!s || !out) return fail(MTS_E_ARG, "null argument");
+  mts::RefStops r;
+  mts::reference_stops(s->res, r);
+  std::memset(out, 0, sizeof(*out));
+  for (int k = 0; k < 4; k++) out->count[k] = r.count[k];
+  out->first_kind = r.first_kind;
+  out->first_dir = r.first_dir;
+  out->first_index = r.first_index;
+  std::snprintf(out->message, sizeof out->message, "%s", r.message.c_str());
+  return MTS_OK;
+}
 
 int mts_setup_create(const mts_grid_desc *grid, const mts_surface_desc *surface,
                      const mts_physics_desc *physics, int mib, int accuracy, mts_setup **out) {
@@ -887,7 +928,7 @@ static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::L
     a.tiles[c] = solve ? in->tiles[dir][c].p : nullptr;
     a.ntiles[c] = solve ? (int)in->tiles[dir][c].n : 0;
   }
-  return mts::launch_line(dir, flags, in->f32(), a, st);
+  return mts::launch_line(dir, flags, in->f32(), a, st, &in->side);
 }
 
 int mts_step(mts_instance *in, double t_next) {
@@ -905,7 +946,6 @@ int mts_step(mts_instance *in, double t_next) {
     r1.R = in->r.p;
     r1.dt = dt;
     r1.e = e1;
-    MTS_HIP(timed(in, "adi_rhs1", [&] { return launch_rhs1(r1, st); }));
     SpecArgs sp;
     sp.g = in->geom;
     sp.P = in->phys;
@@ -916,7 +956,7 @@ int mts_step(mts_instance *in, double t_next) {
     sp.R = in->r.p;
     sp.dt = dt;
     sp.e = e1;
-    MTS_HIP(timed(in, "adi_spec", [&] { return launch_spec(sp, st); }));
+    MTS_HIP(timed(in, "adi_rhs1", [&] { return launch_rhs1(r1, sp, st); }));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;

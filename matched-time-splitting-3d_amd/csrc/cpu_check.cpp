@@ -557,13 +557,34 @@ int main(int argc, char **argv) {
   a.theta_dt = tdt;
   a.dt = dt;
   if (adi) {
+    // the stage-1 launch: every (x row, node pair) the pair threads cover,
+    // the MIB-affected nodes (classification +-2, as the instance plan marks
+    // them) left to spec_node; the pair form must equal the plain per-node
+    // form bit for bit, and the spec nodes must be exactly the ones it skips
+    for (size_t r = 0; r < E.t.spec.size(); r += kSpecLen) {
+      int8_t &v = E.ind[((size_t)E.t.spec[r + 2] * nn[1] + E.t.spec[r + 1]) * E.px + E.t.spec[r]];
+      v = v > 0 ? 2 : -2;
+    }
     Rhs1Args r1{E.g, E.t.phys, E.u.data(), E.r.data(), dt, e1};
+    const double kUnset = -12345.5;
+    std::fill(E.r.begin(), E.r.end(), kUnset);
+    const unsigned hp = (unsigned)E.px / 2;
+    for (unsigned row = 0; row < (unsigned)(nn[1] * nn[2]); row++)
+      for (unsigned pp = 0; pp < hp; pp++) rhs1_pair(r1, row, pp);
+    std::vector<double> plain(E.fe, 0.0);
     for (int iz = 0; iz < nn[2]; iz++)
       for (int iy = 0; iy < nn[1]; iy++)
         for (int ix = 0; ix < nn[0]; ix++) {
-          rhs1_node(r1, ix, iy, iz);
+          const size_t c = node_index(E.g, ix, iy, iz);
+          const bool spec = E.ind[c] == 2 || E.ind[c] == -2;
           const double v = rhs1_plain(r1, ix, iy, iz);
-          if (std::memcmp(&v, &E.r[node_index(E.g, ix, iy, iz)], sizeof v) != 0) {
+          if (spec ? E.r[c] != kUnset : std::memcmp(&v, &E.r[c], sizeof v) != 0) {
+            std::printf("rhs1 pair at node (%d, %d, %d) differs from the plain form\n", ix, iy,
+                        iz);
+            return 1;
+          }
+          rhs1_node(r1, ix, iy, iz);
+          if (std::memcmp(&v, &E.r[c], sizeof v) != 0) {
             std::printf("rhs1 node (%d, %d, %d) differs from the plain form\n", ix, iy, iz);
             return 1;
           }

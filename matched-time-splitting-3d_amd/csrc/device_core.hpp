@@ -34,6 +34,19 @@ namespace mts {
 MTS_HD double fmad(double a, double b, double c) { return fma(a, b, c); }
 MTS_HD float fmad(float a, float b, float c) { return fmaf(a, b, c); }
 
+// Two neighbouring doubles at a 16-byte aligned address, as one load on the
+// device.
+MTS_HD void ld2(const double *a, double (&v)[2]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const double2 t = *reinterpret_cast<const double2 *>(a);
+  v[0] = t.x;
+  v[1] = t.y;
+#else
+  v[0] = a[0];
+  v[1] = a[1];
+#endif
+}
+
 // ------------------------------------------------------------ geometry
 MTS_HD double tabv(const DevGeom &g, int a, int kind, int i) {
   return g.tab[a][kind * g.n[a] + i];
@@ -260,6 +273,21 @@ struct Rhs1Trig {
     return kind == kTabSin ? s[a] : kind == kTabCos ? c[a] : kind == kTabCosM ? cym : cyp;
   }
 };
+// R at an interior node from its value, its y and z neighbours and the
+// node's trigonometric values (douglas_adi.cc:93-359 operation order).
+MTS_HD double rhs1_value(const PhysConst &P, double dt, double e, bool out, const Rhs1Trig &t,
+                         double u, double uym, double uyp, double uzm, double uzp) {
+  double vy[3], vz[3];
+  op_weights_with(P, 1, out, t, vy);
+  op_weights_with(P, 2, out, t, vz);
+  double sum_y = uym * vy[0] + u * vy[1] + uyp * vy[2];
+  double sum_z = uzm * vz[0] + u * vz[1] + uzp * vz[2];
+  double r = dt * sum_y;
+  r += dt * sum_z;
+  r += u;
+  r += dt * source_with(P, e, out, t);
+  return r;
+}
 MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
   const DevGeom &g = p.g;
   const size_t c = node_index(g, ix, iy, iz);
@@ -282,17 +310,60 @@ MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
     p.R[c] = outer_with(p.e, t);
     return;
   }
-  const bool out = ind > 0;
-  double vy[3], vz[3];
-  op_weights_with(p.P, 1, out, t, vy);
-  op_weights_with(p.P, 2, out, t, vz);
-  double sum_y = uym * vy[0] + u * vy[1] + uyp * vy[2];
-  double sum_z = uzm * vz[0] + u * vz[1] + uzp * vz[2];
-  double r = p.dt * sum_y;
-  r += p.dt * sum_z;
-  r += u;
-  r += p.dt * source_with(p.P, p.e, out, t);
-  p.R[c] = r;
+  p.R[c] = rhs1_value(p.P, p.dt, p.e, ind > 0, t, u, uym, uyp, uzm, uzp);
+}
+
+// The same for the node pair (2 pp, 2 pp + 1) of x row `row` (= iz * ny +
+// iy), pp < px / 2: one 16-byte load per field row, one 16-byte store when
+// both nodes are written.  Nodes flagged as MIB-affected (|ind| == 2, spec_node
+// writes them) and the x padding are left alone.
+MTS_HD void rhs1_pair(const Rhs1Args &p, unsigned row, unsigned pp) {
+  const DevGeom &g = p.g;
+  const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
+  const int iy = (int)(row % (unsigned)n1), iz = (int)(row / (unsigned)n1);
+  const int ix0 = 2 * (int)pp;
+  if (ix0 >= n0) return;
+  const bool two = ix0 + 1 < n0;
+  const bool bnd_row = iy == 0 || iz == 0 || iy == n1 - 1 || iz == n2 - 1;
+  const size_t c = node_index(g, ix0, iy, iz);
+  const size_t sy = bnd_row ? 0 : (size_t)g.px, sz = bnd_row ? 0 : (size_t)n1 * g.px;
+  // (the pitch is even and a whole number of 16-byte units, so c is 16-byte
+  // aligned; the second element of a pair past the row's end is padding)
+  double u[2], uym[2], uyp[2], uzm[2], uzp[2];
+  ld2(p.u + c, u);
+  ld2(p.u + c - sy, uym);
+  ld2(p.u + c + sy, uyp);
+  ld2(p.u + c - sz, uzm);
+  ld2(p.u + c + sz, uzp);
+  const signed char i0 = g.ind[c], i1 = g.ind[c + 1];
+  Rhs1Trig t;
+  t.s[1] = tabv(g, 1, kTabSin, iy);
+  t.s[2] = tabv(g, 2, kTabSin, iz);
+  t.c[1] = tabv(g, 1, kTabCos, iy);
+  t.c[2] = tabv(g, 2, kTabCos, iz);
+  t.cym = tabv(g, 1, kTabCosM, iy);
+  t.cyp = tabv(g, 1, kTabCosP, iy);
+  const double sx[2] = {tabv(g, 0, kTabSin, ix0), two ? tabv(g, 0, kTabSin, ix0 + 1) : 0.0};
+  const double cx[2] = {tabv(g, 0, kTabCos, ix0), two ? tabv(g, 0, kTabCos, ix0 + 1) : 0.0};
+  double r[2];
+  bool w[2];
+  for (int q = 0; q < 2; q++) {
+    const int ix = ix0 + q;
+    const signed char ind = q == 0 ? i0 : i1;
+    w[q] = ix < n0 && ind != 2 && ind != -2;
+    t.s[0] = sx[q];
+    t.c[0] = cx[q];
+    const bool bnd = bnd_row || ix == 0 || ix == n0 - 1;
+    r[q] = bnd ? outer_with(p.e, t)
+               : rhs1_value(p.P, p.dt, p.e, ind > 0, t, u[q], uym[q], uyp[q], uzm[q], uzp[q]);
+  }
+  if (w[0] && w[1]) {
+    p.R[c] = r[0];
+    p.R[c + 1] = r[1];
+  } else {
+    if (w[0]) p.R[c] = r[0];
+    if (w[1]) p.R[c + 1] = r[1];
+  }
 }
 
 // Douglas stage-1 right-hand side at the t-th MIB-affected node: the

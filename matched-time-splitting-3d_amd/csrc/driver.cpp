@@ -20,6 +20,9 @@
 #include <getopt.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <iomanip>
 #include <chrono>
 #include <cstdio>
 #include <fstream>
@@ -93,12 +96,14 @@ Config read_config(const std::string &file) {
     val.erase(std::remove(val.begin(), val.end(), ' '), val.end());
     val.erase(std::remove(val.begin(), val.end(), '\r'), val.end());
     try {
-      if (key == "x-min") c.dom[0] = std::stod(val);
-      else if (key == "x-max") c.dom[1] = std::stod(val);
-      else if (key == "y-min") c.dom[2] = std::stod(val);
-      else if (key == "y-max") c.dom[3] = std::stod(val);
-      else if (key == "z-min") c.dom[4] = std::stod(val);
-      else if (key == "z-max") c.dom[5] = std::stod(val);
+      // the domain bounds are read as integers, as the reference reads them
+      // (stoi into doubles, configuration/read_config.cc:39-56): "3.99" is 3
+      if (key == "x-min") c.dom[0] = std::stoi(val);
+      else if (key == "x-max") c.dom[1] = std::stoi(val);
+      else if (key == "y-min") c.dom[2] = std::stoi(val);
+      else if (key == "y-max") c.dom[3] = std::stoi(val);
+      else if (key == "z-min") c.dom[4] = std::stoi(val);
+      else if (key == "z-max") c.dom[5] = std::stoi(val);
       else if (key == "x-grid") c.n[0] = std::stoi(val);
       else if (key == "y-grid") c.n[1] = std::stoi(val);
       else if (key == "z-grid") c.n[2] = std::stoi(val);
@@ -196,6 +201,45 @@ std::unique_ptr<Surface_Cartesian> make_surface(const Config &c) {
   if (c.surface == "ellipsoid") return std::unique_ptr<Surface_Cartesian>(new Surface_Ellipsoid(c.surface_args));
   if (c.surface == "torus") return std::unique_ptr<Surface_Cartesian>(new Surface_Torus(c.surface_args));
   return std::unique_ptr<Surface_Cartesian>(new Surface_Bumpy(c.surface_args));
+}
+
+// Write_txt (helper.h:442-575), which the reference's main calls after every
+// run: for each crossing (z lines, then y, then x, each in the reference's
+// loop order) the node its ID's parity picks (left_loc for odd |ID|, else
+// left_loc + 1), with its value and its error against the outer exact
+// solution at the configuration's finish time, into "Numerical
+// Solutions.txt" and "Numerical Error.txt" in the working directory.  The
+// reference's Equation pointer there dangles (it outlives the object it
+// points to); here the equation is the run's.
+void write_txt(const Intersections &inter, const Mesh &mesh, Beta &beta,
+               const CubicDoub &uh, double t) {
+  std::ofstream sol("Numerical Solutions.txt", std::ios::out);
+  std::ofstream err("Numerical Error.txt", std::ios::out);
+  if (!sol.good() || !err.good()) fatal("No such file is found!");
+  sol << std::setiosflags(std::ios::right) << std::setprecision(6) << std::fixed;
+  err << std::setiosflags(std::ios::right) << std::setprecision(6) << std::fixed;
+  const Equation eq(t, beta);
+  auto one = [&](const mts_crossing &c, int ix, int iy, int iz) {
+    const int node = (std::abs(c.id) % 2 == 1) ? c.left_loc : c.left_loc + 1;
+    if (c.dir == 2) iz = node;
+    else if (c.dir == 1) iy = node;
+    else ix = node;
+    const double v = uh[ix][iy][iz];
+    const double e = std::abs(v - eq.Outer_u(mesh.xi[ix], mesh.yi[iy], mesh.zi[iz]));
+    sol << std::setw(5) << ix << std::setw(5) << iy << std::setw(5) << iz << std::setw(20) << v
+        << std::endl;
+    err << std::setw(5) << ix << std::setw(5) << iy << std::setw(5) << iz << std::setw(20) << e
+        << std::endl;
+  };
+  for (int ix = 1; ix < mesh.nx - 1; ix++)
+    for (int iy = 1; iy < mesh.ny - 1; iy++)
+      for (const mts_crossing &c : inter.ifpz[ix][iy]) one(c, ix, iy, 0);
+  for (int ix = 1; ix < mesh.nx - 1; ix++)
+    for (int iz = 1; iz < mesh.nz - 1; iz++)
+      for (const mts_crossing &c : inter.ifpy[ix][iz]) one(c, ix, 0, iz);
+  for (int iy = 1; iy < mesh.ny - 1; iy++)
+    for (int iz = 1; iz < mesh.nz - 1; iz++)
+      for (const mts_crossing &c : inter.ifpx[iy][iz]) one(c, 0, iy, iz);
 }
 
 void print_help(const char *bin) {
@@ -323,6 +367,7 @@ int main(int argc, char *argv[]) {
   const std::chrono::duration<double> period = std::chrono::high_resolution_clock::now() - t_start;
   out << std::setprecision(1) << std::endl;
   out << "CPU time cost: " << period.count() << " seconds" << std::endl << std::endl;
+  write_txt(inter, mesh, beta, uh, c.t1);
   std::cout << "[INFO]: PROGRAM FINISHED!" << std::endl;
   return EXIT_SUCCESS;
 }

@@ -158,25 +158,30 @@ __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int
 }
 
 // ------------------------------------------------ Douglas stage-1 RHS
-// One thread per node, nodes in x-fastest order, 256 per workgroup.  The
-// hardware deals workgroups to the 8 XCDs round-robin; workgroup b is given
-// node block (b mod 8) nper + b / 8, so each XCD walks one contiguous run of
-// z-planes and the y and z neighbours a block reads were fetched into that
-// XCD's L2 by the blocks just before it.
+// One launch: the first nper * 8 workgroups compute R on node pairs (one
+// thread per pair of neighbouring x nodes, pairs in x-fastest order with
+// px / 2 per x row, 256 per workgroup), the rest on the MIB-affected nodes
+// (spec_node), which the pair threads leave alone.  The hardware deals
+// workgroups to the 8 XCDs round-robin; pair workgroup b is given pair block
+// (b mod 8) nper + b / 8, so each XCD walks one contiguous run of z-planes
+// and the y and z neighbours a block reads were fetched into that XCD's L2 by
+// the blocks just before it.
 constexpr int kXcds = 8;
-__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, unsigned nodes, unsigned nper) {
+__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsigned npairs,
+                                                   unsigned nper, unsigned hp, double inv_hp) {
+  if (blockIdx.x >= nper * kXcds) {
+    const int t = (int)(blockIdx.x - nper * kXcds) * 256 + (int)threadIdx.x;
+    if (t < sp.nspec) spec_node(sp, t);
+    return;
+  }
   const unsigned b = (blockIdx.x % kXcds) * nper + blockIdx.x / kXcds;
   const unsigned t = b * 256u + threadIdx.x;
-  if (t >= nodes) return;
-  const unsigned n0 = (unsigned)p.g.n[0], n1 = (unsigned)p.g.n[1];
-  const unsigned r = t / n0;
-  rhs1_node(p, (int)(t - r * n0), (int)(r % n1), (int)(r / n1));
-}
-
-__global__ __launch_bounds__(256) void spec_kernel(SpecArgs p) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= p.nspec) return;
-  spec_node(p, t);
+  if (t >= npairs) return;
+  // t / hp through a double reciprocal, corrected to the exact quotient
+  unsigned row = (unsigned)((double)t * inv_hp);
+  if (row * hp > t) row--;
+  else if ((row + 1) * hp <= t) row++;
+  rhs1_pair(p, row, t - row * hp);
 }
 
 // ------------------------------------------------------- boundary faces
@@ -393,8 +398,9 @@ hipError_t launch_norms_scratch(const DevGeom &g, const void *u, bool f32, doubl
 }
 
 // --------------------------------------------------------------- launchers
-hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipStream_t s) {
-  auto one = [&](const LineArgs &in, int min_lanes) -> hipError_t {
+hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipStream_t s,
+                       const SideStream *side) {
+  auto one = [&](const LineArgs &in, int min_lanes, hipStream_t s) -> hipError_t {
     LineArgs a = in;
     if (!make_line_part(a.g.n[dir], a.q, min_lanes)) return hipErrorInvalidValue;
     if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
@@ -412,25 +418,35 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
     LineArgs ar = args, ag = args;
     ar.ntiles[1] = 0;
     ag.ntiles[0] = 0;
+    const int gl = args.gen_lanes > args.min_lanes ? args.gen_lanes : args.min_lanes;
+    if (side && side->s && ag.ntiles[1] > 0 && ar.ntiles[0] > 0) {
+      // both kinds at once: the few, long general tiles first, on the side
+      // stream, and the regular tiles beside them
+      hipError_t e = hipEventRecord(side->fork, s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(side->s, side->fork, 0);
+      if (e == hipSuccess) e = one(ag, gl, side->s);
+      if (e == hipSuccess) e = one(ar, args.min_lanes, s);
+      if (e == hipSuccess) e = hipEventRecord(side->join, side->s);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, side->join, 0);
+      return e;
+    }
     hipError_t e = hipSuccess;
-    if (ag.ntiles[1] > 0) e = one(ag, args.gen_lanes > args.min_lanes ? args.gen_lanes : args.min_lanes);
-    if (e == hipSuccess && ar.ntiles[0] > 0) e = one(ar, args.min_lanes);
+    if (ag.ntiles[1] > 0) e = one(ag, gl, s);
+    if (e == hipSuccess && ar.ntiles[0] > 0) e = one(ar, args.min_lanes, s);
     return e;
   }
-  return one(args, args.min_lanes);
+  return one(args, args.min_lanes, s);
 }
 
-hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s) {
-  const size_t nodes = (size_t)a.g.n[0] * a.g.n[1] * a.g.n[2];
-  if (nodes == 0 || nodes > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
-  const unsigned nper = (unsigned)(((nodes + 255) / 256 + kXcds - 1) / kXcds);
-  hipLaunchKernelGGL(rhs1_kernel, dim3(nper * kXcds), dim3(256), 0, s, a, (unsigned)nodes, nper);
-  return hipGetLastError();
-}
-
-hipError_t launch_spec(const SpecArgs &a, hipStream_t s) {
-  if (a.nspec == 0) return hipSuccess;
-  hipLaunchKernelGGL(spec_kernel, dim3((a.nspec + 255) / 256), dim3(256), 0, s, a);
+hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, hipStream_t s) {
+  if (a.g.px % 2 != 0 || sp.nspec < 0) return hipErrorInvalidValue;
+  const unsigned hp = (unsigned)(a.g.px / 2);
+  const size_t pairs = (size_t)hp * a.g.n[1] * a.g.n[2];
+  if (pairs == 0 || pairs > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
+  const unsigned nper = (unsigned)(((pairs + 255) / 256 + kXcds - 1) / kXcds);
+  const unsigned nspec_blocks = (unsigned)((sp.nspec + 255) / 256);
+  hipLaunchKernelGGL(rhs1_kernel, dim3(nper * kXcds + nspec_blocks), dim3(256), 0, s, a, sp,
+                     (unsigned)pairs, nper, hp, 1.0 / hp);
   return hipGetLastError();
 }
 

@@ -10,9 +10,20 @@
 
 namespace mts {
 
+// A second stream and two events: a solve stage split into general and
+// regular tiles runs the general tiles on s while the regular ones run on the
+// stage's stream (fork: s waits for the work before the stage; join: the
+// stage's stream waits for s).  The two tile sets touch disjoint lines.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
 // One line stage along direction dir with the given flags; f32: the fields
-// in LineArgs are FP32 (LOD and TS stages only).
-hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &a, hipStream_t s);
+// in LineArgs are FP32 (LOD and TS stages only).  side (may be null): where
+// the general tiles of a split stage run.
+hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &a, hipStream_t s,
+                       const SideStream *side = nullptr);
 
 // Per direction and lane-group size (line_kernels.hip); a.q already filled.
 #define MTS_DECL_DL(D, LL)                                                              \
@@ -23,11 +34,10 @@ MTS_DECL_DL(1, 16) MTS_DECL_DL(1, 32) MTS_DECL_DL(1, 64)
 MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
 #undef MTS_DECL_DL
 
-// Douglas stage 1 right-hand side (see device_core.hpp).
-hipError_t launch_rhs1(const Rhs1Args &a, hipStream_t s);
-
-// Douglas stage 1 at MIB-affected nodes (see device_core.hpp).
-hipError_t launch_spec(const SpecArgs &a, hipStream_t s);
+// Douglas stage 1 right-hand side (see device_core.hpp): the regular
+// stencil on every node but the MIB-affected ones (classification +-2), and
+// those from sp (spec_node), in one launch.
+hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, hipStream_t s);
 
 // Boundary faces: out = exact outer solution at e (+ cf * f(e) when cf != 0);
 // f32: out is an FP32 field.

@@ -170,6 +170,21 @@ def test_driver_dry_run_writes_configuration(tmp_path):
     assert "crossings x" in p.stdout
 
 
+def test_driver_reads_domain_bounds_as_integers(tmp_path):
+    """The reference reads x-max etc. with stoi (configuration/read_config.cc:39-56), so
+    a fractional bound is truncated: 1.9 runs the box of 1."""
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    p1, out1 = _driver(tmp_path / "a", CFG, "--dry-run")
+    p2, out2 = _driver(tmp_path / "b", CFG.replace("x-max = 1\n", "x-max = 1.9\n"), "--dry-run")
+    assert p1.returncode == 0 and p2.returncode == 0, p2.stderr
+    assert out1 == out2
+
+    def crossings(stdout):
+        return [l for l in stdout.splitlines() if "crossings" in l]
+    assert crossings(p1.stdout) == crossings(p2.stdout) != []
+
+
 @pytest.mark.parametrize("key,val,msg", [
     ("equation", "3", "not available in this build"),
     ("beta", "3", "not available in this build"),

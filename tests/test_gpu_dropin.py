@@ -30,6 +30,39 @@ def test_cpp_solver_classes():
     assert not bad, bad
 
 
+def check_write_txt(tmp_path, g):
+    """The result files the reference's main writes after every run (Write_txt,
+    helper.h:442-575): per crossing, z lines then y then x, each in the
+    reference's loop order (the line's first fixed index outer), the node its
+    ID's parity picks, its value and its error against the outer exact
+    solution at the configuration's finish time; "%5d%5d%5d%20.6f" lines.
+    The values are the reference's own final field (the GPU's agrees to far
+    below the printed 6 decimals)."""
+    meta = g["meta"]
+    n1 = meta["n"] + 1
+    xs = -1.0 + np.arange(n1) * (2.0 / (n1 - 1.0))
+    e = np.exp(-meta["T"])
+    want = []
+    for d in (2, 1, 0):
+        ii = g[f"cr{d}_int"]
+        order = sorted(range(len(ii)), key=lambda q: (ii[q][0], ii[q][1]))
+        for q in order:
+            j, k, cid, ll = (int(v) for v in ii[q][:4])
+            node = ll if abs(cid) % 2 == 1 else ll + 1
+            ix, iy, iz = (node, j, k) if d == 0 else (j, node, k) if d == 1 else (j, k, node)
+            u = float(g["u_final"][iz, iy, ix])
+            ex = e * np.cos(xs[ix]) * np.cos(xs[iy]) * np.cos(xs[iz])
+            want.append((ix, iy, iz, u, abs(u - ex)))
+    for fname, col in (("Numerical Solutions.txt", 3), ("Numerical Error.txt", 4)):
+        lines = (tmp_path / fname).read_text().splitlines()
+        assert len(lines) == len(want)
+        for line, w in zip(lines, want):
+            assert len(line) == 35 and line[15:].startswith(" ")
+            got = (int(line[0:5]), int(line[5:10]), int(line[10:15]))
+            assert got == w[:3]
+            assert abs(float(line[15:]) - w[col]) <= 1.01e-6
+
+
 def test_driver_reproduces_the_reference_report(tmp_path):
     g = load_golden("cfg0_adi_mib1")
     meta = g["meta"]
@@ -46,10 +79,11 @@ def test_driver_reproduces_the_reference_report(tmp_path):
     out = tmp_path / "out.txt"
     print("mts_driver cfg0", flush=True)
     p = subprocess.run([os.path.join(BUILD, "mts_driver"), "-i", str(cfg), "-o", str(out)],
-                       capture_output=True, text=True, timeout=240)
+                       capture_output=True, text=True, timeout=240, cwd=tmp_path)
     assert p.returncode == 0, p.stdout + p.stderr
     text = out.read_text()
     assert "Error of Douglas-ADI with MIB" in text
+    check_write_txt(tmp_path, g)
     # the interface diagnostics, as the reference's report lines
     rep = [f"{v:.3e}" for v in g["diag_report"]]
     assert f"Error of FPs ------------------------\nLmax = {rep[0]}\nL2   = {rep[1]}" in text
@@ -95,7 +129,7 @@ def test_driver_other_methods_match_the_reference(tmp_path, name):
     out = tmp_path / "out.txt"
     print("mts_driver", name, flush=True)
     p = subprocess.run([os.path.join(BUILD, "mts_driver"), "-i", str(cfg), "-o", str(out)],
-                       capture_output=True, text=True, timeout=240)
+                       capture_output=True, text=True, timeout=240, cwd=tmp_path)
     assert p.returncode == 0, p.stdout + p.stderr
     text = out.read_text()
     linf = float(g["summary"][2])
