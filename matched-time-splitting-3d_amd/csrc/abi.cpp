@@ -143,6 +143,7 @@ struct mts_instance {
   int method = 0;
   int min_lanes = 16;  // lane-group lower bound of the line split (tuning)
   int gen_lanes = 16;  // lanes per line of the general tiles' solve (tuning)
+  bool douglas_fused = true;  // Douglas stage 1 inside the x sweep (kFlagPre1)
   double dt = 0.0, theta_dt = 0.0;
   mts::Grid grid;
   int px = 0;
@@ -166,7 +167,9 @@ struct mts_instance {
   DevBuf<uint8_t> cls0[3];
   // stage records per (direction, stage kind)
   std::map<int, std::unique_ptr<DevBuf<double>>> recs;
-  static int rec_key(int d, int flags) { return d * 64 + (flags & ~(mts::kFlagEpiF | mts::kFlagSeq)); }
+  static int rec_key(int d, int flags) {
+    return d * 64 + (flags & ~(mts::kFlagEpiF | mts::kFlagSeq | mts::kFlagPre1));
+  }
   const double *rec_for(int d, int flags) const {
     auto it = recs.find(rec_key(d, flags));
     return it == recs.end() ? nullptr : it->second->p;
@@ -476,6 +479,7 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
   in->grid = g;
   in->min_lanes = p.min_lanes;
   in->gen_lanes = p.gen_lanes;
+  if (const char *ev = std::getenv("MTS_DOUGLAS_FUSED")) in->douglas_fused = std::atoi(ev) != 0;
   in->px = p.px;
   in->field_elems = p.field_elems;
   MTS_HIP(hipStreamCreateWithFlags(&in->sh.s, hipStreamNonBlocking));
@@ -956,14 +960,28 @@ int mts_step(mts_instance *in, double t_next) {
     sp.R = in->r.p;
     sp.dt = dt;
     sp.e = e1;
-    MTS_HIP(timed(in, "adi_rhs1", [&] { return launch_rhs1(r1, sp, st); }));
+    // Fused (the default): the x sweep computes the stage-1 right-hand side
+    // of u^n as it stages its tiles (kFlagPre1), so only the MIB-affected
+    // nodes are computed beforehand (into r, where the x sweep reads them) and
+    // R is never written out and read back.  MTS_DOUGLAS_FUSED=0: the whole
+    // right-hand side into r first, then the x sweep over r.
+    const bool fused = in->douglas_fused;
+    MTS_HIP(timed(in, fused ? "adi_spec" : "adi_rhs1",
+                  [&] { return launch_rhs1(r1, sp, !fused, st); }));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
     a.mib = in->mib[0];
-    a.A = in->r.p;
+    a.A = fused ? in->u.p : in->r.p;
+    a.S = fused ? in->r.p : nullptr;
     a.out = in->r.p;
-    MTS_HIP(timed(in, "adi_x", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve, a, st); }));
+    a.in_analytic = fused ? 1 : 0;  // the x sweep's end rows: boundary values at t^{n+1}
+    a.e_in = e1;
+    MTS_HIP(timed(in, "adi_x", [&] {
+      return line_launch(in, 0, kRhsCopy | kFlagSolve | (fused ? kFlagPre1 : 0), a, st);
+    }));
+    a.in_analytic = 0;
+    a.A = in->r.p;
     a.mib = in->mib[1];
     a.S = in->u.p;
     MTS_HIP(timed(in, "adi_y", [&] { return line_launch(in, 1, kRhsDouglas | kFlagSolve, a, st); }));

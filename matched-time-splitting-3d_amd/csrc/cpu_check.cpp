@@ -89,6 +89,7 @@ void emulate_line(Emu &E, const LineArgs &p) {
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;
+  constexpr bool PRE1 = DIR == 0 && (FLAGS & kFlagPre1) != 0;
   constexpr int T = tile_lines(L);
   const LinePart &q = p.q;
   const int n = p.g.n[DIR];
@@ -111,6 +112,23 @@ void emulate_line(Emu &E, const LineArgs &p) {
         for (int u = 0; u < kTileSteps; u++) {
           const WalkPair t = w.at(p, u);
           check_pair<DIR, T>(p, w, t, j0, k);
+          if (PRE1) {
+            // the x sweep's tile of u^n staged as the stage-1 right-hand side
+            // (interior lines only), pair by pair as the kernel does
+            if (!(t.ld0 && t.w0) && !(t.ld1 && t.w1)) continue;
+            Pre1Pair e;
+            pre1_load(p, t.gi, t.r, e);
+            const Pre1Line ln = pre1_line(p.g, w.j, k);
+            if (t.ld0 && t.w0) {
+              tA[t.sl0] = pre1_value(p, ln, e, t.gi, 0);
+              seen[t.sl0]++;
+            }
+            if (t.ld1 && t.w1) {
+              tA[t.sl1] = pre1_value(p, ln, e, t.gi, 1);
+              seen[t.sl1]++;
+            }
+            continue;
+          }
           if (!t.ld0 && !t.ld1) continue;
           const double *fA = static_cast<const double *>(p.A);
           const double *fS = static_cast<const double *>(p.S);
@@ -140,9 +158,11 @@ void emulate_line(Emu &E, const LineArgs &p) {
           tile_end_value<DIR>(p, j0, k, e, tA.get());
           seen[sl]++;
         }
-      for (int l = 0; l < T && j0 + l < nj; l++)
+      for (int l = 0; l < T && j0 + l < nj; l++) {
+        if (PRE1 && (j0 + l < 1 || j0 + l > nj - 2)) continue;  // (not staged)
         for (int r = 0; r < n; r++)
           if (seen[(size_t)l * q.lds_len + lds_slot(q, r)] != 1) std::abort();
+      }
       std::unique_ptr<double[]> wt(new double[(size_t)T * kWtLen]);
       for (int e = 0; e < T * kWtLen; e++) line_weight_entry<DIR>(p, j0, k, e, wt.get());
       std::vector<double> zt(2 * (size_t)p.g.n[2]);
@@ -601,12 +621,39 @@ int main(int argc, char **argv) {
     sp.e = e1;
     for (int t = 0; t < sp.nspec; t++) spec_node(sp, t);
     if (dump) dump_field(dump_dir + "/rhs1.bin", E.r, nn, E.px);
+    // the fused x sweep (the default): r holds only the MIB-affected nodes'
+    // values, the sweep stages u^n as R; its output must equal the separate
+    // stage-1 pass followed by the plain x sweep, bit for bit
+    std::vector<double> r_fused(E.fe, -7.5);
+    for (size_t r = 0; r < E.t.spec.size(); r += kSpecLen) {
+      const size_t c = node_index(E.g, E.t.spec[r], E.t.spec[r + 1], E.t.spec[r + 2]);
+      r_fused[c] = E.r[c];
+    }
     a.e_out = e1;
     a.e_jump = e1;
+    a.mib = E.mib[0];
+    {
+      LineArgs af = a;
+      af.A = E.u.data();
+      af.S = r_fused.data();
+      af.out = r_fused.data();
+      af.in_analytic = 1;
+      af.e_in = e1;
+      line<kRhsCopy | kFlagSolve | kFlagPre1>(E, 0, af);
+    }
     a.A = E.r.data();
     a.out = E.r.data();
-    a.mib = E.mib[0];
     line<kRhsCopy | kFlagSolve>(E, 0, a);
+    for (int iz = 1; iz < nn[2] - 1; iz++)
+      for (int iy = 1; iy < nn[1] - 1; iy++)
+        for (int ix = 0; ix < nn[0]; ix++) {
+          const size_t c = node_index(E.g, ix, iy, iz);
+          if (std::memcmp(&r_fused[c], &E.r[c], sizeof(double)) != 0) {
+            std::printf("fused x sweep differs at node (%d, %d, %d): %.17g vs %.17g\n", ix, iy,
+                        iz, r_fused[c], E.r[c]);
+            return 1;
+          }
+        }
     if (dump) dump_field(dump_dir + "/ustar.bin", E.r, nn, E.px);
     a.S = E.u.data();
     a.mib = E.mib[1];

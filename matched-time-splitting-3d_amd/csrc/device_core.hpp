@@ -560,6 +560,23 @@ MTS_HD void tile_end_value(const LineArgs &p, int j0, int k, int e, R *tA) {
   const int sl = tile_end_slot<DIR>(p, j0, e);
   if (sl >= 0) tA[sl] = (R)tile_end_val<DIR>(p, j0, k, e);
 }
+// The Douglas stage-1 right-hand side computed inside the x sweep
+// (kFlagPre1): the sweep loads u^n and turns every interior element of its
+// tile into R as it stages it, so R is never written to or read from memory.
+// Along an x line (j = iy, k = iz) the y and z factors are fixed.
+struct Pre1Line {
+  double sy, cy, cym, cyp, sz, cz;
+};
+MTS_HD Pre1Line pre1_line(const DevGeom &g, int j, int k) {
+  Pre1Line l;
+  l.sy = tabv(g, 1, kTabSin, j);
+  l.cy = tabv(g, 1, kTabCos, j);
+  l.cym = tabv(g, 1, kTabCosM, j);
+  l.cyp = tabv(g, 1, kTabCosP, j);
+  l.sz = tabv(g, 2, kTabSin, k);
+  l.cz = tabv(g, 2, kTabCos, k);
+  return l;
+}
 // Elements per thread that cover a tile: T * n <= 17 * 256 for every split
 // make_line_part returns.
 constexpr int kTileElsPerThread = 17;
@@ -658,6 +675,49 @@ struct TileWalk {
     return w;
   }
 };
+
+// The loads of one element pair (rows r, r + 1 of the line, global index gi,
+// 16-byte aligned): u^n, its y and z neighbours, the classification and the
+// x tables.  Only for pairs of interior lines (the neighbours exist).
+struct Pre1Pair {
+  double u[2], ym[2], yp[2], zm[2], zp[2], sx[2], cx[2];
+  signed char ind[2];
+};
+MTS_HD void pre1_load(const LineArgs &p, size_t gi, int r, Pre1Pair &e) {
+  const DevGeom &g = p.g;
+  const double *u = static_cast<const double *>(p.A);
+  const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
+  ld2(u + gi, e.u);
+  ld2(u + gi - sy, e.ym);
+  ld2(u + gi + sy, e.yp);
+  ld2(u + gi - sz, e.zm);
+  ld2(u + gi + sz, e.zp);
+  e.ind[0] = g.ind[gi];
+  e.ind[1] = g.ind[gi + 1];
+  const int n = g.n[0];
+  e.sx[0] = tabv(g, 0, kTabSin, r);
+  e.cx[0] = tabv(g, 0, kTabCos, r);
+  e.sx[1] = r + 1 < n ? tabv(g, 0, kTabSin, r + 1) : 0.0;
+  e.cx[1] = r + 1 < n ? tabv(g, 0, kTabCos, r + 1) : 0.0;
+}
+// R of element q of the pair: rhs1_value in the stage-1 launch's operation
+// order (the same value, bit for bit), or at a MIB-affected node
+// (classification +-2) the value spec_node wrote into p.S.
+MTS_HD double pre1_value(const LineArgs &p, const Pre1Line &l, const Pre1Pair &e, size_t gi,
+                         int q) {
+  if (e.ind[q] == 2 || e.ind[q] == -2) return static_cast<const double *>(p.S)[gi + q];
+  Rhs1Trig t;
+  t.s[0] = e.sx[q];
+  t.c[0] = e.cx[q];
+  t.s[1] = l.sy;
+  t.c[1] = l.cy;
+  t.s[2] = l.sz;
+  t.c[2] = l.cz;
+  t.cym = l.cym;
+  t.cyp = l.cyp;
+  return rhs1_value(p.P, p.dt, p.e_in, e.ind[q] > 0, t, e.u[q], e.ym[q], e.yp[q], e.zm[q],
+                    e.zp[q]);
+}
 
 // Per-line table entries a lane needs, read once up front.
 struct LineMeta {

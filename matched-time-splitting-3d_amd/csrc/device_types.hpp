@@ -132,6 +132,7 @@ enum : int {
   kFlagEpiF = 16,     // epilogue: x + dt f(t)  on every row (LOD last sweep)
   kFlagSeq = 32,      // some lines need the sequential Thomas solve (DevMib::seq)
   kFlagFull = 64,     // every segment but a line's last has exactly mmax rows (set at launch)
+  kFlagPre1 = 128,    // Douglas x sweep: the tile holds the stage-1 right-hand side of u^n
 };
 
 struct LineArgs {

@@ -438,13 +438,14 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
   return one(args, args.min_lanes, s);
 }
 
-hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, hipStream_t s) {
+hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, hipStream_t s) {
   if (a.g.px % 2 != 0 || sp.nspec < 0) return hipErrorInvalidValue;
   const unsigned hp = (unsigned)(a.g.px / 2);
-  const size_t pairs = (size_t)hp * a.g.n[1] * a.g.n[2];
-  if (pairs == 0 || pairs > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
+  const size_t pairs = pairs_too ? (size_t)hp * a.g.n[1] * a.g.n[2] : 0;
+  if (pairs > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
   const unsigned nper = (unsigned)(((pairs + 255) / 256 + kXcds - 1) / kXcds);
   const unsigned nspec_blocks = (unsigned)((sp.nspec + 255) / 256);
+  if (nper * kXcds + nspec_blocks == 0) return hipSuccess;
   hipLaunchKernelGGL(rhs1_kernel, dim3(nper * kXcds + nspec_blocks), dim3(256), 0, s, a, sp,
                      (unsigned)pairs, nper, hp, 1.0 / hp);
   return hipGetLastError();

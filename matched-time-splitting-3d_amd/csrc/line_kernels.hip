@@ -366,6 +366,37 @@ __device__ __forceinline__ TileWalk<DIR, T> fresh_walk(const LineArgs &p, int j0
   return TileWalk<DIR, T>(p, j0, k, tid);
 }
 
+// Douglas x sweep (kFlagPre1): stage the tile of u^n as the stage-1
+// right-hand side.  CHUNK pairs at a time, each pair loading its four y / z
+// neighbour pairs, its classification and its x tables, all in flight
+// together: in the general kernel 3 with 16-row segments (its solve needs
+// more registers than the chunk anyway), else 1 (3 waves per SIMD); in the
+// regular kernel 1, which keeps it within 128 VGPRs at 4 workgroups per CU
+// (2 at 3 workgroups per CU measured 130 us against 107 at 257^3).  End rows come from tile_end_val; pairs of lines past
+// the grid or on its boundary are not staged (those lines are not solved).
+template <int T, int CHUNK>
+__device__ __forceinline__ void pre1_stage_tile(const LineArgs &p, int j0, int k, int tid,
+                                                double *tA) {
+  const TileWalk<0, T> walk(p, j0, k, tid);
+  const Pre1Line ln = pre1_line(p.g, walk.j < p.g.n[1] ? walk.j : 0, k);
+#pragma unroll 1
+  for (int u0 = 0; u0 < kTilePairsPerThread; u0 += CHUNK) {
+    Pre1Pair e[CHUNK];
+    WalkPair tp[CHUNK];
+#pragma unroll
+    for (int c = 0; c < CHUNK; c++) {
+      tp[c] = walk.at(p, u0 + c);
+      if (u0 + c >= kTilePairsPerThread) tp[c].ld0 = tp[c].ld1 = false;  // (last chunk)
+      if ((tp[c].ld0 && tp[c].w0) || (tp[c].ld1 && tp[c].w1)) pre1_load(p, tp[c].gi, tp[c].r, e[c]);
+    }
+#pragma unroll
+    for (int c = 0; c < CHUNK; c++) {
+      if (tp[c].ld0 && tp[c].w0) tA[tp[c].sl0] = pre1_value(p, ln, e[c], tp[c].gi, 0);
+      if (tp[c].ld1 && tp[c].w1) tA[tp[c].sl1] = pre1_value(p, ln, e[c], tp[c].gi, 1);
+    }
+  }
+}
+
 // One tile of one line stage: every row built from the tables (regular rows
 // and MIB blocks), the general partitioned solve.
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
@@ -429,6 +460,10 @@ __device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
   const int end_sl = end_row ? tile_end_slot<DIR>(p, j0, tid) : -1;
   const double end_v = end_sl >= 0 ? tile_end_val<DIR>(p, j0, k, tid) : 0.0;
 
+  constexpr bool PRE1 = DIR == 0 && (FLAGS & kFlagPre1) != 0;
+  if constexpr (PRE1) {
+    pre1_stage_tile<T, (MMAX >= 16 ? 3 : 1)>(p, j0, k, tid, reinterpret_cast<double *>(tA));
+  } else {
   const TileWalk<DIR, T> walk(p, j0, k, tid);
 #pragma unroll 1
   for (int u0 = 0; u0 < kTilePairsPerThread; u0 += kChunk) {
@@ -457,6 +492,7 @@ __device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
         if (HAS_S) tB[t.sl1] = vb[c].y;
       }
     }
+  }
   }
   double *wt = reinterpret_cast<double *>(tA + (size_t)T * q.lds_len * (TWO ? 2 : 1));
   double *zt = ZT ? wt + T * kWtLen : nullptr;
@@ -622,6 +658,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
             16, 0, 0);
     }
   }
+  constexpr bool PRE1 = DIR == 0 && (FLAGS & kFlagPre1) != 0;
+  if constexpr (PRE1) {
+    pre1_stage_tile<T, 1>(p, cur.j0, cur.k, tid, reinterpret_cast<double *>(tA));
+  } else {
   const TileWalk<DIR, T> walk(p, cur.j0, cur.k, tid);
   V2 va[kChunk], vb[kChunk];
 #pragma unroll
@@ -646,6 +686,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
       tA[w.sl1] = va[c].y;
       if (HAS_S) tB[w.sl1] = vb[c].y;
     }
+  }
   }
   if (tid < T * kWtLen) wt[tid] = cur.wt_v;
   if (cur.end_sl >= 0) tA[cur.end_sl] = (R)cur.end_v;
@@ -710,7 +751,6 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
   }
 }
-
 // A solve stage with tile lists runs the regular tiles with line_kernel_reg
 // and the others with line_kernel; any other stage runs line_kernel over the
 // whole grid.
@@ -795,6 +835,9 @@ hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream
   }
   switch (flags) {
     MTS_LINE_CASE(kRhsCopy | kFlagSolve, double)
+#if MTS_LINE_DIR == 0
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagPre1, double)
+#endif
     MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, double)
     MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, double)
     MTS_LINE_CASE(kRhsDouglas | kFlagSolve, double)

@@ -73,7 +73,7 @@ void fill(const SetupResult &s, const DeviceTableHost &t, const LineArgs &p0,
 template <int DIR>
 bool dispatch(int flags, const SetupResult &s, const DeviceTableHost &t, const LineArgs &p,
               std::vector<double> &rec) {
-  switch (flags & ~(kFlagEpiF | kFlagSeq)) {
+  switch (flags & ~(kFlagEpiF | kFlagSeq | kFlagPre1)) {
     case kRhsCopy | kFlagSolve: fill<DIR, kRhsCopy | kFlagSolve>(s, t, p, rec); return true;
     case kRhsCopy | kFlagSolve | kFlagLhsJump:
       fill<DIR, kRhsCopy | kFlagSolve | kFlagLhsJump>(s, t, p, rec);
