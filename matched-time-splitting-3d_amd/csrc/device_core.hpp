@@ -440,7 +440,7 @@ inline bool make_line_part(int n, LinePart &q, int min_lanes = 16) {
       q.p = p;
       q.stride = mx | 1;
       q.magic = ((1 << 20) + m - 1) / m;
-      q.lds_len = 2 + p * q.stride;
+      q.lds_len = (2 + p * q.stride) | 1;  // odd: neighbouring lines on other banks
       return true;
     }
   }

@@ -50,14 +50,24 @@ enum : int { kRegR = 0, kRegA = 1, kRegCf = 2, kRegAf = 3, kRegCfin = 4, kRegRow
 //   q     reciprocal of the lane's final reduced pivot
 enum : int {
   kRegCD = 0, kRegI1 = 1, kRegA1 = 2, kRegC1 = 3, kRegP1 = 4, kRegP2 = 5, kRegK = 6,
-  kRegMaxLevels = 6, kRegQ = kRegK + 2 * kRegMaxLevels, kRegScal = kRegQ + 2,
+  kRegMaxLevels = 6, kRegQ = kRegK + 2 * kRegMaxLevels, kRegScal = kRegQ + 3,
 };
 
 // Table of one direction and side: the common segment (every lane but the
-// last) and the last segment's row factors, then every lane's scalars.
-MTS_RS_HD int reg_side_len(int L, int mmax) { return 2 * mmax * kRegRow + L * kRegScal; }
-MTS_RS_HD int reg_rows_off(int variant, int mmax) { return variant * mmax * kRegRow; }
-MTS_RS_HD int reg_scal_off(int lane, int mmax) { return 2 * mmax * kRegRow + lane * kRegScal; }
+// last) and the last segment's row factors, then every lane's scalars.  The
+// layout keeps the reads of one instruction on different LDS banks: the
+// lanes of a group read one row of the common segment (a broadcast) and the
+// last lane the same row of its own segment, kRegPad doubles off the bank of
+// the first; each lane reads the same scalar of its own block, an odd number
+// of doubles (kRegScal) apart.
+constexpr int kRegPad = 2;
+MTS_RS_HD int reg_side_len(int L, int mmax) {  // (even: L is a power of two >= 16)
+  return 2 * mmax * kRegRow + 2 * kRegPad + L * kRegScal;
+}
+MTS_RS_HD int reg_rows_off(int variant, int mmax) { return variant * (mmax * kRegRow + kRegPad); }
+MTS_RS_HD int reg_scal_off(int lane, int mmax) {
+  return 2 * mmax * kRegRow + 2 * kRegPad + lane * kRegScal;
+}
 
 // Downward and upward elimination of the lane's right-hand side d[0..m)
 // (FULL: m == MMAX, padding rows hold d = 0).  Returns the segment's reduced

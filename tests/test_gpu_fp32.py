@@ -90,9 +90,10 @@ def test_fp32_footprint_and_serving(mts, ctx):
     # field plus the node classification byte; the regular lines' solve
     # tables in the field's type (65-node lines: 16-lane groups of 4-row
     # segments, guarded and unguarded tables, the outside side only (beta
-    # varies inside), three directions: 3 * 2 * (2 * 4 * 6 + 16 * 20)
-    # elements, whole 16-byte units either way); the rest the same
-    tables = 3 * 2 * (2 * 4 * 6 + 16 * 20)
+    # varies inside), three directions: 3 * 2 * (2 * 4 * 6 + 4 + 16 * 21)
+    # elements: two 4-row segments of 6 factors, 2 + 2 padding, 16 lanes of
+    # 21 scalars; whole 16-byte units either way); the rest the same
+    tables = 3 * 2 * (2 * 4 * 6 + 4 + 16 * 21)
     assert b64 - 80 * 65 * 65 * 9 - 8 * tables == b32 - 96 * 65 * 65 * 5 - 4 * tables
     i32.initialize(0.0)
     steps, t_end = i32.run(0.0, case[6] * case[7])
