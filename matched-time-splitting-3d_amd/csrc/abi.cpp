@@ -427,8 +427,10 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
       if (!ok) return fail(MTS_E_ARG, "zero pivot in the regular lines' line solve");
       const int nj = d == 0 ? g.n[1] : g.n[0];
       const int nk = d == 2 ? g.n[1] : g.n[2];
-      mts::stage_tiles(t.blk_ptr[d].data(), t.cls0[d].data(), t.phys.beta_kind == 0, g.n[d], nj,
-                       nk, p->min_lanes, p->gen_lanes, p->tiles[d][0], p->tiles[d][1]);
+      mts::stage_tiles(t.blk_ptr[d].data(), t.cls0[d].data(), t.phys.beta_kind == 0, d, g.n[d],
+                       nj, nk, p->min_lanes, p->gen_lanes,
+                       precision == MTS_FP32 ? (int)sizeof(float) : (int)sizeof(double),
+                       p->tiles[d][0], p->tiles[d][1]);
     }
     // exactly what instance_from_plan allocates
     size_t b = p->field_elems * ((precision == MTS_FP32 ? sizeof(float) : sizeof(double)) +

@@ -519,13 +519,45 @@ int main(int argc, char **argv) {
     if (const char *ev = std::getenv("MTS_LINE_LANES")) ml = std::atoi(ev);
     for (int d = 0; d < 3; d++) {
       const int nj = d == 0 ? nn[1] : nn[0], nk = d == 2 ? nn[1] : nn[2];
-      std::vector<int> lr, lg;
-      if (!stage_tiles(E.t.blk_ptr[d].data(), E.t.cls0[d].data(), E.t.phys.beta_kind == 0, nn[d], nj,
-                       nk, ml, 32, lr, lg))
-        std::abort();
       LinePart qr, qg;
       make_line_part(nn[d], qr, ml);
       make_line_part(nn[d], qg, std::max(ml, 32));
+      for (int eb : {8, 4}) {  // the lists of the FP64 and the FP32 form
+      std::vector<int> lr, lg;
+      if (!stage_tiles(E.t.blk_ptr[d].data(), E.t.cls0[d].data(), E.t.phys.beta_kind == 0, d,
+                       nn[d], nj, nk, ml, 32, eb, lr, lg))
+        std::abort();
+      // grouped by line: in every full window of 8 line groups, the tiles
+      // of one line group are 8 entries apart (same XCD)
+      for (int c = 0; c < 2; c++) {
+        const std::vector<int> &lst = c == 0 ? lr : lg;
+        const int G = line_group(d, tile_lines(c == 0 ? qr.L : qg.L), eb);
+        if (G <= 1) continue;
+        // full line groups in the list (all G tiles of a line in it)
+        const int ng = (nj + G - 1) / G + 1;
+        std::vector<int> cnt((size_t)nk * ng, 0);
+        for (int e : lst)
+          if (tile_k(e) < nk && tile_bx(e) / G < ng) cnt[(size_t)tile_k(e) * ng + tile_bx(e) / G]++;
+        size_t full = 0;
+        for (int v : cnt) full += v == G ? 1 : 0;
+        size_t i = 0, windows = 0;
+        while (i + 8 * (size_t)G <= lst.size()) {
+          bool window = true;  // a window of 8 full line groups?
+          for (int u = 0; u < 8 && window; u++)
+            for (int m = 1; m < G; m++) {
+              const int a = lst[i + u], b = lst[i + (size_t)m * 8 + u];
+              if (tile_k(a) != tile_k(b) || tile_bx(a) / G != tile_bx(b) / G) window = false;
+            }
+          if (!window) break;  // the rest: partial line groups and the tail
+          i += 8 * (size_t)G;
+          windows++;
+        }
+        if (windows != full / 8) {
+          std::printf("dir %d: tile list not grouped by line (%zu windows, %zu full groups)\n", d,
+                      windows, full);
+          return 1;
+        }
+      }
       std::vector<int> cover((size_t)nj * nk, 0);
       for (int c = 0; c < 2; c++) {
         const int T = tile_lines(c == 0 ? qr.L : qg.L);
@@ -551,8 +583,9 @@ int main(int argc, char **argv) {
             std::printf("dir %d: line (%d, %d) in %d tiles\n", d, j, k, cover[(size_t)k * nj + j]);
             return 1;
           }
-      std::printf("dir %d tiles: %zu regular (L %d), %zu general (L %d)\n", d, lr.size(), qr.L,
-                  lg.size(), qg.L);
+      std::printf("dir %d tiles (%d-byte elements): %zu regular (L %d), %zu general (L %d)\n", d,
+                  eb, lr.size(), qr.L, lg.size(), qg.L);
+      }
     }
   }
   // time factors: the first step from t = 0 (mts_step with t_next = dt)

@@ -286,7 +286,7 @@ __device__ __forceinline__ void group_thomas(R (&a)[MMAX], R (&b)[MMAX], R (&c)[
 // tile narrower than 128 bytes is one of 128 / (T * sizeof(R)) per line.
 template <int DIR, int T, typename R>
 __device__ __host__ constexpr unsigned tile_group() {
-  return (DIR != 0 && T * sizeof(R) < 128) ? 128u / (T * sizeof(R)) : 1u;
+  return (unsigned)line_group(DIR, T, (int)sizeof(R));
 }
 
 // Pair of neighbouring field elements moved as one access.
@@ -338,22 +338,22 @@ struct LdsLayout {
 // the line is read from HBM once and the other parts come from that XCD's
 // L2.  (Otherwise neighbouring tiles stay on different XCDs at the same time:
 // giving each XCD a contiguous range of tiles instead is slower in every
-// direction.)
+// direction.)  Tile lists come in that order already (xcd_group_tiles).
 template <int DIR, int T, typename R>
 __device__ __forceinline__ void tile_coords(const int *list, int &bx, int &by) {
   constexpr unsigned G = tile_group<DIR, T, R>();
-  const unsigned nb = list ? gridDim.x : gridDim.x * gridDim.y;
-  const unsigned b = list ? blockIdx.x : blockIdx.y * gridDim.x + blockIdx.x;
-  unsigned v = b;
-  if (G > 1 && b < nb / (8 * G) * (8 * G)) v = ((b >> 3) / G * 8 + (b & 7)) * G + (b >> 3) % G;
   if (list) {
-    const int e = list[v];
+    const int e = list[blockIdx.x];
     bx = tile_bx(e);
     by = tile_k(e) - 1;
-  } else {
-    bx = (int)(v % gridDim.x);
-    by = (int)(v / gridDim.x);
+    return;
   }
+  const unsigned nb = gridDim.x * gridDim.y;
+  const unsigned b = blockIdx.y * gridDim.x + blockIdx.x;
+  unsigned v = b;
+  if (G > 1 && b < nb / (8 * G) * (8 * G)) v = ((b >> 3) / G * 8 + (b & 7)) * G + (b >> 3) % G;
+  bx = (int)(v % gridDim.x);
+  by = (int)(v / gridDim.x);
 }
 
 // The tile walk of thread tid, formed from inputs the compiler must treat as

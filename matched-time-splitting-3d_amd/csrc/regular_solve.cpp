@@ -203,14 +203,39 @@ void recut_tiles(const std::vector<int> &from, int t_from, int t_to, int nj, std
   }
 }
 
-bool stage_tiles(const int *blk_ptr, const uint8_t *cls0, bool beta_const, int n, int nj, int nk,
-                 int min_lanes, int gen_lanes, std::vector<int> &regular, std::vector<int> &general) {
+void xcd_group_tiles(std::vector<int> &list, int group) {
+  if (group <= 1) return;
+  // line groups (same k, same bx / group) as they come in the list
+  std::vector<std::vector<int>> full, part;
+  for (size_t i = 0; i < list.size();) {
+    const int key_k = tile_k(list[i]), key_b = tile_bx(list[i]) / group;
+    std::vector<int> u;
+    while (i < list.size() && tile_k(list[i]) == key_k && tile_bx(list[i]) / group == key_b)
+      u.push_back(list[i++]);
+    (u.size() == (size_t)group ? full : part).push_back(std::move(u));
+  }
+  std::vector<int> out;
+  out.reserve(list.size());
+  for (size_t u0 = 0; u0 < full.size(); u0 += 8) {
+    const size_t nu = full.size() - u0 < 8 ? full.size() - u0 : 8;
+    for (int m = 0; m < group; m++)
+      for (size_t u = 0; u < nu; u++) out.push_back(full[u0 + u][(size_t)m]);
+  }
+  for (const auto &u : part) out.insert(out.end(), u.begin(), u.end());
+  list.swap(out);
+}
+
+bool stage_tiles(const int *blk_ptr, const uint8_t *cls0, bool beta_const, int dir, int n, int nj,
+                 int nk, int min_lanes, int gen_lanes, int elem_bytes, std::vector<int> &regular,
+                 std::vector<int> &general) {
   LinePart q, qg;
   if (!make_line_part(n, q, min_lanes)) return false;
   if (!make_line_part(n, qg, gen_lanes > min_lanes ? gen_lanes : min_lanes)) return false;
   std::vector<int> gen;
   tile_lists(blk_ptr, cls0, beta_const, nj, nk, q.L, regular, gen);
   recut_tiles(gen, tile_lines(q.L), tile_lines(qg.L), nj, general);
+  xcd_group_tiles(regular, line_group(dir, tile_lines(q.L), elem_bytes));
+  xcd_group_tiles(general, line_group(dir, tile_lines(qg.L), elem_bytes));
   return true;
 }
 

@@ -140,11 +140,26 @@ void tile_lists(const int *blk_ptr, const uint8_t *cls0, bool beta_const, int nj
 // lines becomes T_from / T_to tiles of T_to lines (T_to divides T_from),
 // leaving out those past the last line nj - 1.
 void recut_tiles(const std::vector<int> &from, int t_from, int t_to, int nj, std::vector<int> &out);
-// Both lists of one solve stage along a direction of n nodes: the regular
+// Tiles that share one 128-byte line: y/z tiles are T consecutive x, so a
+// tile of T elements of elem_bytes narrower than 128 bytes is one of
+// 128 / (T elem_bytes) per line (1: whole lines).
+MTS_RS_HD constexpr int line_group(int dir, int T, int elem_bytes) {
+  return (dir != 0 && T * elem_bytes < 128) ? 128 / (T * elem_bytes) : 1;
+}
+// A tile list reordered so that the tiles of one line group (same k, same
+// bx / group) are 8 entries apart: the workgroups of a launch go to the 8
+// XCDs in turn, so those tiles run on one XCD, dispatched one after the
+// other, and the line is read from HBM once (the others find it in that
+// XCD's L2).  Groups of eight full line groups first, then the groups with
+// tiles in the other list, in order.  A permutation: every tile stays.
+void xcd_group_tiles(std::vector<int> &list, int group);
+// Both lists of one solve stage along direction dir of n nodes: the regular
 // tiles at the split of min_lanes lanes, the general ones at the split of
-// max(gen_lanes, min_lanes) lanes.  False when n has no split.
-bool stage_tiles(const int *blk_ptr, const uint8_t *cls0, bool beta_const, int n, int nj, int nk,
-                 int min_lanes, int gen_lanes, std::vector<int> &regular, std::vector<int> &general);
+// max(gen_lanes, min_lanes) lanes, each grouped by line for elements of
+// elem_bytes (xcd_group_tiles).  False when n has no split.
+bool stage_tiles(const int *blk_ptr, const uint8_t *cls0, bool beta_const, int dir, int n, int nj,
+                 int nk, int min_lanes, int gen_lanes, int elem_bytes, std::vector<int> &regular,
+                 std::vector<int> &general);
 MTS_RS_HD int tile_bx(int e) { return e & 0xffff; }
 MTS_RS_HD int tile_k(int e) { return e >> 16; }
 

@@ -42,9 +42,11 @@ def test_kernel_indices_in_bounds(method, mib, geom):
     t0 = time.time()
     p = subprocess.run([TOOL, method, mib] + args, capture_output=True, text=True, timeout=300)
     assert p.returncode == 0, p.stdout + p.stderr
-    # the tile lists are checked first (one line per direction), then the kernels
+    # the tile lists are checked first (one line per direction and element
+    # size: coverage, classes and the grouping by 128-byte line), then the
+    # kernels
     lines = p.stdout.splitlines()
-    assert sum(l.startswith("dir ") and " tiles: " in l for l in lines) == 3, p.stdout
+    assert sum(l.startswith("dir ") and " tiles (" in l for l in lines) == 6, p.stdout
     assert lines[-1].startswith("ok"), p.stdout
     assert time.time() - t0 < 300
 
