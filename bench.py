@@ -67,6 +67,8 @@ def main():
     ap.add_argument("--hbm-cap-gb", type=float, default=200.0, help="serve: admission cap")
     ap.add_argument("--n-max", type=int, default=256, help="serve: largest N drawn")
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--tune-steps", type=int, default=10,
+                    help="steps the line-split tuner times per split")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
                     help="batch/serve: the form of the LOD and TS solves (Douglas-ADI stays FP64)")
     args = ap.parse_args()
@@ -114,6 +116,12 @@ def main():
     info = st.info()
     ctx = mts.Context(0)
     dev_name, _ = ctx.device_info()
+    # the line split: every split that fits, timed on a scratch instance
+    # (before and outside the timed region); the instance below runs the
+    # fastest
+    t0 = time.perf_counter()
+    split = ctx.tune_lanes(st, args.method, dt, "fp64", steps=args.tune_steps)
+    tune_s = time.perf_counter() - t0
     inst = mts.Instance(ctx, st, args.method, dt)
 
     # warmup: untimed steps from the initial condition
@@ -221,6 +229,8 @@ def main():
             "device": dev_name,
             "stage_ms_per_launch": {k: v[1] / max(v[0], 1) for k, v in stages.items()},
             "graph_ms_per_step": graph_ms,
+            "line_split": {"min_lanes": split[0], "gen_lanes": split[1],
+                           "tuner_ms_per_step": split[2], "tuner_s": tune_s},
         },
         "roofline": {
             "bound": "hbm",
@@ -245,6 +255,7 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf, fp64_diverge
     formed in FP64), with its L-inf error against the exact solution.  When
     either form diverged (the L-inf ratio then compares overflow artefacts),
     the ratios are null and "diverged" says so."""
+    split = ctx.tune_lanes(st, args.method, dt, "fp32", steps=args.tune_steps)
     inst = mts.Instance(ctx, st, args.method, dt, precision="fp32")
     inst.initialize(0.0)
     t = 0.0
@@ -281,6 +292,8 @@ def fp32_form(mts, ctx, st, args, dt, nodes, fp64_value, fp64_linf, fp64_diverge
         "unit": "grid-pt-steps/s",
         "ms_per_step": elapsed / args.steps * 1e3,
         "dtype": "f32 fields, rows and solve; f64 MIB rows, setup and norms",
+        "line_split": {"min_lanes": split[0], "gen_lanes": split[1],
+                       "tuner_ms_per_step": split[2]},
         "linf_error": linf,
         "l2_error": l2,
         "diverged": diverged,

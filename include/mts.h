@@ -198,6 +198,22 @@ int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physi
                            int method, double dt, int precision, mts_instance **out);
 int mts_instance_precision(const mts_instance *inst, int *precision);
 
+/* Line-split autotuner.  No counterpart in the reference (its line solves are
+ * sequential Thomas sweeps).  Times `steps` steps (after two untimed ones,
+ * from the initial field) of this setup, method, dt and precision on a
+ * scratch instance for each line split that fits the grid: lane groups of 16,
+ * 32 or 64 lanes for every line (min_lanes) and 16, 32 or 64 for the general
+ * tiles (gen_lanes >= min_lanes), and returns the fastest.  The choice is kept
+ * in ctx for this grid, method and precision: every later instance created
+ * in ctx (mts_instance_create*, the server) uses it.  Every split computes the
+ * same solution to rounding.  Without a choice instances use 16 / 16, or the
+ * MTS_LINE_LANES / MTS_GEN_LANES environment overrides. */
+int mts_tune_lanes(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                   int method, double dt, int precision, int steps, int *min_lanes,
+                   int *gen_lanes, double *ms_per_step);
+/* The line split an instance runs (min_lanes, gen_lanes). */
+int mts_instance_lanes(const mts_instance *inst, int *min_lanes, int *gen_lanes);
+
 /* Field transfer (host <-> the instance's solution field). */
 int mts_field_upload(mts_instance *inst, const double *host, int layout);
 int mts_field_download(mts_instance *inst, double *host, int layout);

@@ -150,6 +150,8 @@ def lib() -> C.CDLL:
             "mts_instance_dt": ([VP, P(D)], I),
             "mts_instance_create_ex": ([VP, VP, P(PhysicsDesc), I, D, I, P(VP)], I),
             "mts_instance_precision": ([VP, P(I)], I),
+            "mts_tune_lanes": ([VP, VP, P(PhysicsDesc), I, D, I, I, P(I), P(I), P(D)], I),
+            "mts_instance_lanes": ([VP, P(I), P(I)], I),
             "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
             "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
             "mts_batch_run_graph": ([VP, I, VP, VP, VP, VP], I),
@@ -322,6 +324,17 @@ class Context:
         _check(lib().mts_ctx_device_info(self._h, name, 256, C.byref(hbm)))
         return name.value.decode(), hbm.value
 
+    def tune_lanes(self, setup: "Setup", method, dt: float, precision="fp64", steps: int = 5):
+        """Time every line split that fits the grid on a scratch instance and
+        keep the fastest for later instances of this grid, method and
+        precision in this context: (min_lanes, gen_lanes, ms_per_step)."""
+        m = METHODS[method] if isinstance(method, str) else int(method)
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        ml, gl, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(lib().mts_tune_lanes(self._h, setup.handle, C.byref(setup.physics), m, float(dt),
+                                    prec, int(steps), C.byref(ml), C.byref(gl), C.byref(ms)))
+        return ml.value, gl.value, ms.value
+
     def tdma_batched(self, a, b, c, r) -> np.ndarray:
         """Solve nlines tridiagonal systems (arrays shaped (nlines, n))."""
         a, b, c, r = (np.ascontiguousarray(x, dtype=np.float64) for x in (a, b, c, r))
@@ -406,6 +419,12 @@ class Instance:
         _check(lib().mts_profile_read(self._h, buf, 64, C.byref(cnt)))
         return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms)
                 for i in range(min(cnt.value, 64))}
+
+    def lanes(self):
+        """The line split the instance runs: (min_lanes, gen_lanes)."""
+        ml, gl = C.c_int(), C.c_int()
+        _check(lib().mts_instance_lanes(self._h, C.byref(ml), C.byref(gl)))
+        return ml.value, gl.value
 
     def device_bytes(self) -> int:
         """Device memory the instance holds."""

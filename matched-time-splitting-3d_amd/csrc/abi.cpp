@@ -5,12 +5,15 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -136,7 +139,20 @@ struct mts_ctx {
   int device = 0;
   hipDeviceProp_t prop;
   void *norms_scratch = nullptr;  // device address of the library's fixed norms array
+  // mts_tune_lanes' choices: (nx, ny, nz, method, precision) -> split
+  std::mutex tune_mu;
+  std::map<std::array<int, 5>, mts::Lanes> tuned;
 };
+
+bool mts::ctx_tuned_lanes(mts_ctx *ctx, const int n[3], int method, int precision,
+                          mts::Lanes &out) {
+  if (!ctx) return false;
+  std::lock_guard<std::mutex> lk(ctx->tune_mu);
+  auto it = ctx->tuned.find({n[0], n[1], n[2], method, precision});
+  if (it == ctx->tuned.end()) return false;
+  out = it->second;
+  return true;
+}
 
 struct mts_instance {
   mts_ctx *ctx = nullptr;
@@ -333,7 +349,7 @@ void mts::PlanDeleter::operator()(mts::InstancePlan *p) const { delete p; }
 size_t mts::plan_device_bytes(const mts::InstancePlan &p) { return p.device_bytes; }
 
 int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, int method,
-                       double dt, int precision, mts::PlanPtr &out) {
+                       double dt, int precision, mts::PlanPtr &out, const mts::Lanes *lanes) {
   if (!setup || !physics) return fail(MTS_E_ARG, "null argument");
   if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
   if (precision != MTS_FP64 && precision != MTS_FP32) return fail(MTS_E_ARG, "unknown precision");
@@ -363,16 +379,22 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
     p->theta_dt = (method == MTS_ADI || method == MTS_LOD_IE) ? dt : 0.5 * dt;
     p->grid = setup->res.grid;
     const mts::Grid &g = p->grid;
-    // MTS_LINE_LANES=32|64 raises the lane-group lower bound of the line split
-    // (a tuning knob: every split computes the same solution to rounding)
-    if (const char *ev = std::getenv("MTS_LINE_LANES")) {
-      const int v = std::atoi(ev);
-      if (v == 16 || v == 32 || v == 64) p->min_lanes = v;
-    }
-    // MTS_GEN_LANES=32|64: the same for the general tiles of the solve stages
-    if (const char *ev = std::getenv("MTS_GEN_LANES")) {
-      const int v = std::atoi(ev);
-      if (v == 16 || v == 32 || v == 64) p->gen_lanes = v;
+    if (lanes && lanes->min_lanes) {
+      // a chosen split (mts_tune_lanes)
+      p->min_lanes = lanes->min_lanes;
+      p->gen_lanes = lanes->gen_lanes;
+    } else {
+      // MTS_LINE_LANES=32|64 raises the lane-group lower bound of the line
+      // split (every split computes the same solution to rounding)
+      if (const char *ev = std::getenv("MTS_LINE_LANES")) {
+        const int v = std::atoi(ev);
+        if (v == 16 || v == 32 || v == 64) p->min_lanes = v;
+      }
+      // MTS_GEN_LANES=32|64: the same for the general tiles of the solve stages
+      if (const char *ev = std::getenv("MTS_GEN_LANES")) {
+        const int v = std::atoi(ev);
+        if (v == 16 || v == 32 || v == 64) p->gen_lanes = v;
+      }
     }
     for (int d = 0; d < 3; d++) {
       mts::LinePart q;
@@ -803,10 +825,7 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
                         int method, double dt, mts_instance **out) {
   if (!ctx || !out) return fail(MTS_E_ARG, "null argument");
   *out = nullptr;
-  mts::PlanPtr plan;
-  int rc = mts::plan_instance(setup, physics, method, dt, MTS_FP64, plan);
-  if (rc) return rc;
-  return mts::instance_from_plan(ctx, std::move(plan), out);
+  return mts_instance_create_ex(ctx, setup, physics, method, dt, MTS_FP64, out);
 }
 
 int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
@@ -814,9 +833,81 @@ int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physi
   if (!ctx || !out) return fail(MTS_E_ARG, "null argument");
   *out = nullptr;
   mts::PlanPtr plan;
-  int rc = mts::plan_instance(setup, physics, method, dt, precision, plan);
+  mts::Lanes lanes;
+  const bool tuned =
+      setup && mts::ctx_tuned_lanes(ctx, setup->res.grid.n, method, precision, lanes);
+  int rc = mts::plan_instance(setup, physics, method, dt, precision, plan, tuned ? &lanes : nullptr);
   if (rc) return rc;
   return mts::instance_from_plan(ctx, std::move(plan), out);
+}
+
+int mts_instance_lanes(const mts_instance *in, int *min_lanes, int *gen_lanes) {
+  if (!in || !min_lanes || !gen_lanes) return fail(MTS_E_ARG, "null argument");
+  *min_lanes = in->min_lanes;
+  *gen_lanes = in->gen_lanes;
+  return MTS_OK;
+}
+
+int mts_tune_lanes(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
+                   int method, double dt, int precision, int steps, int *min_lanes,
+                   int *gen_lanes, double *ms_per_step) {
+  if (!ctx || !setup || !physics || !min_lanes || !gen_lanes)
+    return fail(MTS_E_ARG, "null argument");
+  if (steps < 1 || steps > 1000) return fail(MTS_E_ARG, "steps must be within 1..1000");
+  static const int kCand[6][2] = {{16, 16}, {16, 32}, {16, 64}, {32, 32}, {32, 64}, {64, 64}};
+  const int *n = setup->res.grid.n;
+  mts::Lanes best;
+  double best_ms = 0.0;
+  int last_rc = MTS_OK;
+  for (const auto &c : kCand) {
+    // splits that do not exist for these lines (longer than 16 or 32 lanes
+    // can hold) are skipped
+    bool ok = true;
+    for (int d = 0; d < 3 && ok; d++) {
+      mts::LinePart q, qg;
+      ok = mts::make_line_part(n[d], q, c[0]) && q.L == c[0] &&
+           mts::make_line_part(n[d], qg, c[1]);
+    }
+    if (!ok) continue;
+    mts::Lanes l;
+    l.min_lanes = c[0];
+    l.gen_lanes = c[1];
+    mts::PlanPtr plan;
+    int rc = mts::plan_instance(setup, physics, method, dt, precision, plan, &l);
+    mts_instance *in = nullptr;
+    if (!rc) rc = mts::instance_from_plan(ctx, std::move(plan), &in);
+    if (rc) {
+      last_rc = rc;
+      if (rc == MTS_E_DISABLED || rc == MTS_E_ARG) return rc;  // (the same for every split)
+      continue;
+    }
+    // from the initial field: two untimed steps, then `steps` timed ones
+    double t = 0.0;
+    rc = mts_initialize(in, 0.0);
+    for (int k = 0; k < 2 && !rc; k++) rc = mts_step(in, t += dt);
+    if (!rc) rc = mts_sync(in);
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < steps && !rc; k++) rc = mts_step(in, t += dt);
+    if (!rc) rc = mts_sync(in);
+    const double ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+        steps;
+    mts_instance_destroy(in);
+    if (rc) return rc;
+    if (!best.min_lanes || ms < best_ms) {
+      best = l;
+      best_ms = ms;
+    }
+  }
+  if (!best.min_lanes) return last_rc ? last_rc : fail(MTS_E_ARG, "no line split fits this grid");
+  {
+    std::lock_guard<std::mutex> lk(ctx->tune_mu);
+    ctx->tuned[{n[0], n[1], n[2], method, precision}] = best;
+  }
+  *min_lanes = best.min_lanes;
+  *gen_lanes = best.gen_lanes;
+  if (ms_per_step) *ms_per_step = best_ms;
+  return MTS_OK;
 }
 
 int mts_instance_precision(const mts_instance *in, int *precision) {

@@ -255,7 +255,13 @@ struct mts_server {
       mts_setup *s = nullptr;
       mts::PlanPtr plan;
       int rc = mts_setup_create(&p.req.grid, &p.req.surface, &p.req.physics, p.req.mib, 2, &s);
-      if (!rc) rc = mts::plan_instance(s, &p.req.physics, p.req.method, p.req.dt, p.req.precision, plan);
+      if (!rc) {
+        mts::Lanes lanes;
+        const bool tuned =
+            mts::ctx_tuned_lanes(ctx, p.req.grid.n, p.req.method, p.req.precision, lanes);
+        rc = mts::plan_instance(s, &p.req.physics, p.req.method, p.req.dt, p.req.precision, plan,
+                                tuned ? &lanes : nullptr);
+      }
       mts_setup_destroy(s);  // the plan holds what the device needs
       std::lock_guard<std::mutex> lk(mu);
       if (rc) {

@@ -7,7 +7,8 @@ Bars, stated here because the form is not the reference's arithmetic:
   its magnitude on the 41^3-65^3 cases (conditioning up to 640), and the
   scaled bound on the long-line grids;
 * the L-infinity error against the exact solution stays within 1.05x of the
-  FP64 form's (the accuracy criterion the FP32 form is measured at).
+  FP64 form's and of the oracle's (the accuracy criterion the FP32 form is
+  measured at).
 """
 import numpy as np
 import pytest
@@ -70,6 +71,52 @@ def test_fp32_error_within_5_percent_of_fp64(mts, ctx, ci):
     (l64, _, _), (l32, _, _) = res
     print(f"  linf FP64 {l64:.6e} FP32 {l32:.6e}", flush=True)
     assert l32 <= 1.05 * l64
+
+
+def _oracle_linf(oracle, st, beta_kind, bp, method, dt, T):
+    op = oracle.OracleProblem.from_setup(st, beta_kind, bp, -1.0, 1.0)
+    u, t_end = op.run(method, dt, 0.0, int(T / dt), op.init(0.0))
+    return op.norms(u, t_end)[0], t_end
+
+
+def test_fp32_error_within_5_percent_of_oracle(mts, oracle, ctx):
+    """The FP32 form's L-infinity error against the exact solution within
+    1.05x of the oracle's (FP64, the reference's algorithm) on the bench
+    workload at 65^3 (LOD-CN MIB-L2, ellipsoid (0.7, 0.5, 0.4), beta- =
+    10(1 + 0.5 sin x cos y), beta+ = 1, dt = h, T = 0.5: 16 steps) and on the
+    smallest LOD-CN solve of the seeded uneven batch (the FP32 form runs no
+    Douglas-ADI)."""
+    import importlib.util
+    import os
+
+    from conftest import ROOT
+
+    spec = importlib.util.spec_from_file_location("mts_bench", os.path.join(ROOT, "bench.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    rng = np.random.default_rng(0)
+    draws = [b.draw_solve(rng, int(rng.choice(np.arange(32, 193, 16))),
+                          ["adi", "lod-cn"][int(rng.integers(2))], 1, 0.5) for _ in range(96)]
+    d = min((d for d in draws if d["method"] == "lod-cn"), key=lambda d: d["n"])
+    cases = [
+        (64, ("ellipsoid", (0.7, 0.5, 0.4, 1 - 2 ** -20)), ("sincos", 10, 1), 1, (10, 1), 2,
+         1 / 32, 0.5),
+        (d["n"], d["surface"], ("const",) + tuple(d["beta"]), 0, d["beta"], d["mib"], d["dt"],
+         d["T"]),
+    ]
+    for n, surf, beta, kind, bp, mib, dt, T in cases:
+        print(f"fp32 vs oracle: N={n} {surf[0]} beta {beta} MIB-L{mib}", flush=True)
+        st = mts.Setup(mts.grid(n + 1), mts.surface(surf[0], *surf[1]), mts.physics(*beta),
+                       mib=mib)
+        lo, t_o = _oracle_linf(oracle, st, kind, bp, "lod-cn", dt, T)
+        inst = mts.Instance(ctx, st, "lod-cn", dt, precision="fp32")
+        inst.initialize(0.0)
+        steps, t_end = inst.run(0.0, T)
+        l32 = inst.error(t_end)[0]
+        inst.close()
+        assert t_end == t_o
+        print(f"  linf oracle {lo:.6e} FP32 {l32:.6e} ({l32 / lo:.5f}x)", flush=True)
+        assert l32 <= 1.05 * lo
 
 
 def test_fp32_douglas_is_refused(mts, ctx):
