@@ -132,6 +132,16 @@ int mts_setup_diagnostics(const mts_setup *s, double t, double fp[2], double jum
 int mts_setup_crossing_errors(const mts_setup *s, int dir, double t, double *errl, double *errr,
                               double *jerr, size_t count);
 
+/* The same diagnostics on the device, from the instance's MIB tables (its
+ * setup's crossings, weights and jump records): one thread per line, then
+ * one workgroup's reduction.  Per crossing bit for bit the host's
+ * (mts_setup_crossing_errors); the maxima equal the host's, the L2 norms
+ * agree to rounding (another summation order).  The first call allocates the
+ * per-crossing buffers (3 doubles per crossing) on the instance. */
+int mts_instance_diagnostics(mts_instance *inst, double t, double fp[2], double jump[2]);
+int mts_instance_crossing_errors(mts_instance *inst, int dir, double t, double *errl,
+                                 double *errr, double *jerr, size_t count);
+
 /* Where the reference itself would have stopped a run that this library
  * completes.  Its set-up prepares numeric tangential flux-jump approximations
  * for every crossing (Auxiliary_eta_* / Auxiliary_tau_*,

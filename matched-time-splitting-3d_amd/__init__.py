@@ -151,6 +151,8 @@ def lib() -> C.CDLL:
             "mts_instance_create_ex": ([VP, VP, P(PhysicsDesc), I, D, I, P(VP)], I),
             "mts_instance_precision": ([VP, P(I)], I),
             "mts_tune_lanes": ([VP, VP, P(PhysicsDesc), I, D, I, I, P(I), P(I), P(D)], I),
+            "mts_instance_diagnostics": ([VP, D, VP, VP], I),
+            "mts_instance_crossing_errors": ([VP, I, D, VP, VP, VP, C.c_size_t], I),
             "mts_instance_lanes": ([VP, P(I), P(I)], I),
             "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
             "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
@@ -419,6 +421,22 @@ class Instance:
         _check(lib().mts_profile_read(self._h, buf, 64, C.byref(cnt)))
         return {buf[i].name.decode(): (buf[i].launches, buf[i].total_ms)
                 for i in range(min(cnt.value, 64))}
+
+    def diagnostics(self, t: float):
+        """The interface diagnostics on the device: ((FP Lmax, FP L2),
+        (jump Lmax, jump L2)) at time t."""
+        fp = np.zeros(2)
+        jp = np.zeros(2)
+        _check(lib().mts_instance_diagnostics(self._h, t, _dptr(fp), _dptr(jp)))
+        return (float(fp[0]), float(fp[1])), (float(jp[0]), float(jp[1]))
+
+    def crossing_errors(self, d: int, t: float):
+        """Per crossing of direction d, on the device: (errl, errr, jump error)."""
+        n = int(self.setup.info().n_cross[d])
+        el, er, je = np.zeros(n), np.zeros(n), np.zeros(n)
+        _check(lib().mts_instance_crossing_errors(self._h, d, t, _dptr(el), _dptr(er),
+                                                  _dptr(je), n))
+        return el, er, je
 
     def lanes(self):
         """The line split the instance runs: (min_lanes, gen_lanes)."""

@@ -197,6 +197,7 @@ struct mts_instance {
   DevBuf<int> spec;
   int nspec = 0;
   DevBuf<double> part_max, part_sq;
+  DevBuf<double> diag;  // interface diagnostics: el, er, je per crossing, then 4 sums (on first use)
   DevBuf<long long> part_idx;
   mts::PhysConst phys;
   mts::DevGeom geom;
@@ -1244,6 +1245,82 @@ int mts_error_norms(mts_instance *in, double t, double *linf, double *l2, int ar
     argmax[1] = (int)((mi / nz) % ny);
     argmax[2] = (int)(mi % nz);
   }
+  return MTS_OK;
+}
+
+// The interface diagnostics of every crossing at time t into in->diag (el
+// at [0, total), er at [total, 2 total), je at [2 total, 3 total), directions
+// one after the other); off[d] is direction d's first crossing.
+static int diag_run(mts_instance *in, double t, int off[4]) {
+  off[0] = 0;
+  for (int d = 0; d < 3; d++) off[d + 1] = off[d] + (int)(in->cr_val[d].n / mts::kCrLen);
+  const int total = off[3];
+  if (total == 0) return MTS_OK;
+  if (!in->diag.p) {
+    MTS_HIP(in->diag.alloc(3 * (size_t)total + 4, in->stream));
+    MTS_HIP(hipMemsetAsync(in->diag.p, 0, (3 * (size_t)total + 4) * sizeof(double), in->stream));
+  }
+  const mts::Grid &g = in->grid;
+  for (int d = 0; d < 3; d++) {
+    mts::DiagArgs a;
+    a.g = in->geom;
+    a.mib = in->mib[d];
+    a.cr = in->cr_val[d].p;
+    a.nj = d == 0 ? g.n[1] : g.n[0];
+    a.nk = d == 2 ? g.n[1] : g.n[2];
+    a.e = std::exp(-t);
+    a.el = in->diag.p + off[d];
+    a.er = in->diag.p + total + off[d];
+    a.je = in->diag.p + 2 * (size_t)total + off[d];
+    if (off[d + 1] > off[d]) MTS_HIP(mts::launch_diagnostics(d, a, in->stream));
+  }
+  return MTS_OK;
+}
+
+int mts_instance_diagnostics(mts_instance *in, double t, double fp[2], double jump[2]) {
+  if (!in || !fp || !jump) return fail(MTS_E_ARG, "null argument");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  int off[4];
+  int rc = diag_run(in, t, off);
+  if (rc) return rc;
+  const int total = off[3];
+  double r[4] = {0.0, 0.0, 0.0, 0.0};
+  if (total > 0) {
+    double *base = in->diag.p;
+    MTS_HIP(mts::launch_diag_reduce(base, base + total, base + 2 * (size_t)total, total,
+                                    base + 3 * (size_t)total, in->stream));
+    MTS_HIP(hipMemcpyAsync(r, base + 3 * (size_t)total, sizeof r, hipMemcpyDeviceToHost,
+                           in->stream));
+    MTS_HIP(hipStreamSynchronize(in->stream));
+  }
+  // the host report's normalisation: two fictitious points and one jump per crossing
+  fp[0] = r[0];
+  fp[1] = std::sqrt(r[1] / (2.0 * total));
+  jump[0] = r[2];
+  jump[1] = std::sqrt(r[3] / (1.0 * total));
+  return MTS_OK;
+}
+
+int mts_instance_crossing_errors(mts_instance *in, int dir, double t, double *errl,
+                                 double *errr, double *jerr, size_t count) {
+  if (!in || !errl || !errr || !jerr) return fail(MTS_E_ARG, "null argument");
+  if (dir < 0 || dir > 2) return fail(MTS_E_ARG, "direction must be 0, 1 or 2");
+  MTS_HIP(hipSetDevice(in->ctx->device));
+  int off[4];
+  int rc = diag_run(in, t, off);
+  if (rc) return rc;
+  const size_t n = (size_t)(off[dir + 1] - off[dir]);
+  if (count != n) return fail(MTS_E_ARG, "count differs from the direction's crossings");
+  if (n == 0) return MTS_OK;
+  const size_t total = (size_t)off[3];
+  const double *base = in->diag.p;
+  MTS_HIP(hipMemcpyAsync(errl, base + off[dir], n * sizeof(double), hipMemcpyDeviceToHost,
+                         in->stream));
+  MTS_HIP(hipMemcpyAsync(errr, base + total + off[dir], n * sizeof(double),
+                         hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipMemcpyAsync(jerr, base + 2 * total + off[dir], n * sizeof(double),
+                         hipMemcpyDeviceToHost, in->stream));
+  MTS_HIP(hipStreamSynchronize(in->stream));
   return MTS_OK;
 }
 

@@ -35,6 +35,7 @@
 #include <vector>
 
 #include "device_core.hpp"
+#include "diagnostics.hpp"
 #include "host_setup.hpp"
 #include "host_table.hpp"
 #include "regular_solve.hpp"
@@ -587,6 +588,45 @@ int main(int argc, char **argv) {
                   eb, lr.size(), qr.L, lg.size(), qg.L);
       }
     }
+  }
+  // the device interface diagnostics (diag_line, one call per line) against
+  // the host diagnostics: equal to 1e-12 (NaN where the host has NaN).  Not
+  // bit for bit: the device reads sin and cos of a crossing from its jump
+  // record, and a compiler may form that record's sin and cos of one
+  // argument with one sincos call, which can differ from sin alone by an ulp
+  // (g++ does here)
+  for (int d = 0; d < 3; d++) {
+    const double t = 0.37;
+    std::vector<double> hl, hr, hj;
+    crossing_errors(res, beta, d, t, hl, hr, hj);
+    const size_t nc = hl.size();
+    std::vector<double> el(nc, -1.0), er(nc, -1.0), je(nc, -1.0);
+    DiagArgs a;
+    a.g = E.g;
+    a.mib = E.mib[d];
+    a.cr = E.t.cr_val[d].empty() ? nullptr : E.t.cr_val[d].data();
+    a.nj = d == 0 ? nn[1] : nn[0];
+    a.nk = d == 2 ? nn[1] : nn[2];
+    a.e = std::exp(-t);
+    a.el = el.data();
+    a.er = er.data();
+    a.je = je.data();
+    for (int lid = 0; lid < a.nj * a.nk; lid++) {
+      if (d == 0) diag_line<0>(a, lid);
+      else if (d == 1) diag_line<1>(a, lid);
+      else diag_line<2>(a, lid);
+    }
+    auto same = [](double x, double y) {
+      if (std::isnan(x) || std::isnan(y)) return std::isnan(x) && std::isnan(y);
+      if (std::isinf(x) || std::isinf(y)) return x == y;
+      return std::fabs(x - y) <= 1e-12;
+    };
+    for (size_t c = 0; c < nc; c++)
+      if (!same(el[c], hl[c]) || !same(er[c], hr[c]) || !same(je[c], hj[c])) {
+        std::printf("dir %d crossing %zu: device diagnostics differ from the host's\n", d, c);
+        std::printf("%.17g %.17g | %.17g %.17g | %.17g %.17g id %d ll %d\n", el[c], hl[c], er[c], hr[c], je[c], hj[c], res.cross[d][c].id, res.cross[d][c].left_loc);
+        return 1;
+      }
   }
   // time factors: the first step from t = 0 (mts_step with t_next = dt)
   const double t1 = dt;

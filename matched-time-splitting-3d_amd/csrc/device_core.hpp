@@ -719,6 +719,104 @@ MTS_HD double pre1_value(const LineArgs &p, const Pre1Line &l, const Pre1Pair &e
                     e.zp[q]);
 }
 
+// ------------------------------------------------ interface diagnostics
+// The exact jumps at crossing c (douglas_adi.cc's equation: u- = e sin sin
+// sin, u+ = e cos cos cos): [u], the GFM's u_dir (P^{-1}[d][0] times row 0 of
+// P times [beta grad u]) and the exact [beta u_d], in the operation order of
+// the host diagnostics and of jumps().
+template <int DIR>
+MTS_HD void diag_jumps(const double *cr, double e, double &ju, double &udir, double &bd) {
+  const double sx = cr[0], sy = cr[1], sz = cr[2], cx = cr[3], cy = cr[4], cz = cr[5];
+  const double bi = cr[6], bo = cr[7];
+  const double me = -e;
+  ju = e * cx * cy * cz - e * sx * sy * sz;
+  const double jbx = bo * (me * sx * cy * cz) - bi * (e * cx * sy * sz);
+  const double jby = bo * (me * cx * sy * cz) - bi * (e * sx * cy * sz);
+  const double jbz = bo * (me * cx * cy * sz) - bi * (e * sx * sy * cz);
+  const double xi = cr[8] * jbx + cr[9] * jby + cr[10] * jbz;
+  udir = cr[11] * xi;
+  bd = DIR == 0 ? jbx : (DIR == 1 ? jby : jbz);
+}
+// The errors of every crossing of line lid (Refresh_Fp + Error_Fp,
+// intersection.cc:1841-2281 and 4667-4771; Refresh_Jump's jump.err and
+// Error_Jump, :2294-2490 and 4594-4665): each fictitious point from its
+// weights (the block record's copies of the crossing's), the exact solution
+// on the line's nodes and the exact jumps, relative to the other side's
+// exact value; the jump error |u_dir - [beta u_d]| / [beta u_d].  A corner
+// pair shares its middle fictitious point (errr of the first crossing is errl
+// of the second).
+template <int DIR>
+MTS_HD void diag_line(const DiagArgs &a, int lid) {
+  const int bb = a.mib.blk_ptr[lid], be = a.mib.blk_ptr[lid + 1];
+  const int j = lid % a.nj, k = lid / a.nj;
+  const DevGeom &g = a.g;
+  const double e = a.e;
+  // the exact value at line node i on its own side (inv: the other side)
+  auto node_u = [&](int i, bool inv) {
+    int ix, iy, iz;
+    LineMap<DIR>::node(j, k, i, ix, iy, iz);
+    const bool out = g.ind[node_index(g, ix, iy, iz)] > 0;
+    const double outer = e * tabv(g, 0, kTabCos, ix) * tabv(g, 1, kTabCos, iy) *
+                         tabv(g, 2, kTabCos, iz);
+    const double inner = e * tabv(g, 0, kTabSin, ix) * tabv(g, 1, kTabSin, iy) *
+                         tabv(g, 2, kTabSin, iz);
+    return out != inv ? outer : inner;
+  };
+  auto rel = [](double ref, double v) { return fabs((ref - v) / ref); };
+  for (int bi = bb; bi < be; bi++) {
+    const BlockHdr h = a.mib.hdr[bi];
+    const double *v = a.mib.val + h.off;
+    const int ll = h.ll, c = h.c0;
+    double juL, udL, bdL;
+    diag_jumps<DIR>(a.cr + (size_t)c * kCrLen, e, juL, udL, bdL);
+    a.je[c] = fabs(udL - bdL) / bdL;
+    if (h.type == kBlkIrr) {
+      const double *wl = v + kIrrWl, *wr = v + kIrrWr;
+      double sl = 0.0, sr = 0.0;
+      sl += wl[4] * juL;
+      sl += wl[5] * bdL;
+      sr += wr[4] * juL;
+      sr += wr[5] * bdL;
+      for (int q = 0; q < 4; q++) {
+        const double u = node_u(ll - 1 + q, false);
+        sl += wl[q] * u;
+        sr += wr[q] * u;
+      }
+      a.el[c] = rel(node_u(ll, true), sl);
+      a.er[c] = rel(node_u(ll + 1, true), sr);
+    } else {
+      double juR, udR, bdR;
+      diag_jumps<DIR>(a.cr + (size_t)(c + 1) * kCrLen, e, juR, udR, bdR);
+      a.je[c + 1] = fabs(udR - bdR) / bdR;
+      const double *lwl = v + kCorLwl, *lwr = v + kCorLwr, *rwr = v + kCorRwr;
+      double lsl = 0.0, lsr = 0.0, rsr = 0.0;
+      lsl += lwl[5] * juL;
+      lsl += lwl[6] * bdL;
+      lsl += lwl[7] * juR;
+      lsl += lwl[8] * bdR;
+      lsr += lwr[5] * juL;
+      lsr += lwr[6] * bdL;
+      lsr += lwr[7] * juR;
+      lsr += lwr[8] * bdR;
+      rsr += rwr[5] * juL;
+      rsr += rwr[6] * bdL;
+      rsr += rwr[7] * juR;
+      rsr += rwr[8] * bdR;
+      for (int q = 0; q < 5; q++) {
+        const double u = node_u(ll - 1 + q, false);
+        lsl += lwl[q] * u;
+        lsr += lwr[q] * u;
+        rsr += rwr[q] * u;
+      }
+      const double mid = rel(node_u(ll + 1, true), lsr);
+      a.el[c] = rel(node_u(ll, true), lsl);
+      a.er[c] = mid;
+      a.el[c + 1] = mid;
+      a.er[c + 1] = rel(node_u(ll + 2, true), rsr);
+    }
+  }
+}
+
 // Per-line table entries a lane needs, read once up front.
 struct LineMeta {
   int bb, be;    // block range

@@ -182,6 +182,17 @@ struct Rhs1Args {
   double dt;
   double e;  // exp(-t^{n+1})
 };
+// Interface diagnostics of one direction at time factor e = exp(-t): per
+// crossing, its two fictitious-point errors (el, er) and its jump error (je),
+// written at the crossing's index (the arrays already offset by direction).
+struct DiagArgs {
+  DevGeom g;
+  DevMib mib;
+  const double *cr;  // jump records, kCrLen per crossing
+  int nj, nk;        // lines of the direction: line id k * nj + j
+  double e;
+  double *el, *er, *je;
+};
 // Douglas stage 1 at MIB-affected nodes: recompute R with the fictitious-point
 // stencils along y and z and the jump terms of all three directions.
 struct SpecArgs {

@@ -300,6 +300,49 @@ __global__ __launch_bounds__(64) void norms_final_kernel(const double *pmax,
   out->argmax = mi;
 }
 
+// ------------------------------------------------ interface diagnostics
+// One thread per line of the direction (lines without blocks return at once).
+template <int DIR>
+__global__ __launch_bounds__(256) void diag_kernel(DiagArgs a) {
+  const int lid = (int)(blockIdx.x * 256 + threadIdx.x);
+  if (lid >= a.nj * a.nk) return;
+  diag_line<DIR>(a, lid);
+}
+// One workgroup: max |x| and the sum of squares of the fictitious-point errors
+// (el, er: n each) and of the jump errors (je), in a fixed order (each thread
+// a fixed stride, then a tree), into out[0..3] = fmax, fsum, jmax, jsum.
+// NaNs drop out of the maxima as in the host loop (a NaN is never larger).
+__global__ __launch_bounds__(1024) void diag_reduce_kernel(const double *el, const double *er,
+                                                           const double *je, int n, double *out) {
+  __shared__ double sh[4][1024];
+  double fm = 0.0, fs = 0.0, jm = 0.0, js = 0.0;
+  for (int i = (int)threadIdx.x; i < n; i += 1024) {
+    const double l = el[i], r = er[i], q = je[i];
+    fs += l * l;
+    fs += r * r;
+    if (fabs(l) > fm) fm = fabs(l);
+    if (fabs(r) > fm) fm = fabs(r);
+    js += q * q;
+    if (fabs(q) > jm) jm = fabs(q);
+  }
+  const int t = (int)threadIdx.x;
+  sh[0][t] = fm;
+  sh[1][t] = fs;
+  sh[2][t] = jm;
+  sh[3][t] = js;
+  __syncthreads();
+  for (int w = 512; w > 0; w >>= 1) {
+    if (t < w) {
+      if (sh[0][t + w] > sh[0][t]) sh[0][t] = sh[0][t + w];
+      sh[1][t] += sh[1][t + w];
+      if (sh[2][t + w] > sh[2][t]) sh[2][t] = sh[2][t + w];
+      sh[3][t] += sh[3][t + w];
+    }
+    __syncthreads();
+  }
+  if (t < 4) out[t] = sh[t][0];
+}
+
 // ----------------------------------------------------- batched tridiagonal
 template <int MMAX>
 __global__ __launch_bounds__(256) void tdma_kernel(int n, int nlines, const double *A,
@@ -490,6 +533,23 @@ hipError_t launch_norms(const DevGeom &g, const void *u, bool f32, double e, dou
   else
     hipLaunchKernelGGL(norms_kernel<double>, dim3(nb), dim3(256), 0, s, g,
                        static_cast<const double *>(u), e, part_max, part_idx, part_sq);
+  return hipGetLastError();
+}
+
+hipError_t launch_diagnostics(int dir, const DiagArgs &a, hipStream_t s) {
+  const int lines = a.nj * a.nk;
+  if (lines <= 0) return hipSuccess;
+  const dim3 grid((unsigned)((lines + 255) / 256));
+  if (dir == 0) hipLaunchKernelGGL(diag_kernel<0>, grid, dim3(256), 0, s, a);
+  else if (dir == 1) hipLaunchKernelGGL(diag_kernel<1>, grid, dim3(256), 0, s, a);
+  else if (dir == 2) hipLaunchKernelGGL(diag_kernel<2>, grid, dim3(256), 0, s, a);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_diag_reduce(const double *el, const double *er, const double *je, int n,
+                              double *out, hipStream_t s) {
+  hipLaunchKernelGGL(diag_reduce_kernel, dim3(1), dim3(1024), 0, s, el, er, je, n, out);
   return hipGetLastError();
 }
 

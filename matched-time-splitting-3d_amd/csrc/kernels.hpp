@@ -70,6 +70,13 @@ hipError_t norms_scratch_ptr(int device, void **ptr);
 hipError_t launch_norms_scratch(const DevGeom &g, const void *u, bool f32, double e,
                                 void *scratch_ptr, hipStream_t s, NormsResult **result);
 
+// Interface diagnostics of direction dir (per crossing, into a.el / a.er /
+// a.je), and their reduction over n crossings into out[0..3] = max and sum of
+// squares of the fictitious-point errors, then of the jump errors.
+hipError_t launch_diagnostics(int dir, const DiagArgs &a, hipStream_t s);
+hipError_t launch_diag_reduce(const double *el, const double *er, const double *je, int n,
+                              double *out, hipStream_t s);
+
 // Batched tridiagonal solve: nlines independent systems of length n, arrays
 // line-major (line l, row i at l*n + i).  a[l*n] and c[l*n + n-1] are ignored.
 hipError_t launch_tdma(int n, int nlines, const double *a, const double *b, const double *c,
