@@ -70,7 +70,7 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=10,
                     help="steps the line-split tuner times per split")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"],
-                    help="batch/serve: the form of the LOD and TS solves (Douglas-ADI stays FP64)")
+                    help="batch/serve: the field precision of the solves")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -194,7 +194,7 @@ def main():
             traffic = tr[key]
 
     fp32 = None
-    if args.fp32_form and args.method != "adi":
+    if args.fp32_form:
         fp32 = fp32_form(mts, ctx, st, args, dt, nodes, value, linf, diverged)
 
     cpu = None
@@ -345,7 +345,7 @@ def bench_batch(args):
 
     def make(d, precision):
         phys = mts.physics("const", *d["beta"])
-        prec = precision if d["method"] != "adi" else "fp64"
+        prec = precision
         try:
             st = mts.Setup(mts.grid(d["n"] + 1), mts.surface(d["surface"][0], *d["surface"][1]),
                            phys, mib=d["mib"])
@@ -388,7 +388,7 @@ def bench_batch(args):
     same = True
     for inst, (steps, t_end), le, te in zip(insts, out, loop_err, ends):
         same &= t_end == te and inst.error(t_end)[0] == le
-    # the same solves batched in the FP32 form (LOD-CN; Douglas-ADI stays FP64)
+    # the same solves batched in the FP32 form
     f32 = None
     if args.precision == "fp32":
         with ThreadPoolExecutor(16) as ex:
@@ -480,7 +480,7 @@ def bench_serve(args):
                                mts.surface(d["surface"][0], *d["surface"][1]),
                                mts.physics("const", *d["beta"]), d["method"], d["dt"], d["T"],
                                mib=d["mib"], tag=i,
-                               precision=args.precision if d["method"] != "adi" else "fp64"))
+                               precision=args.precision))
         if i % 200 == 199:
             print(f"submitted {i + 1}", file=sys.stderr, flush=True)
     results = srv.drain()
@@ -514,7 +514,7 @@ def bench_serve(args):
                         "sphere/ellipsoid/torus, beta-/beta+ log-uniform in [1e-3,1e3], dt=h",
             "hbm_cap_gb": args.hbm_cap_gb,
             "max_inflight": args.max_inflight,
-            "precision": args.precision + " (Douglas-ADI in fp64)",
+            "precision": args.precision,
         },
         "status_counts": status,
         "p50_latency_ms": float(np.percentile(lat, 50)) if len(lat) else None,

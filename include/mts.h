@@ -202,8 +202,8 @@ int mts_instance_create(mts_ctx *ctx, const mts_setup *setup, const mts_physics_
 void mts_instance_destroy(mts_instance *inst);
 /* mts_instance_create with a field precision (MTS_FP64 or MTS_FP32).  No
  * counterpart in the reference (FP64 throughout); the FP32 form halves the
- * field's HBM traffic and registers.  Douglas-ADI is FP64 only
- * (MTS_E_DISABLED). */
+ * field's HBM traffic and registers.  Every method; Douglas-ADI's stage-1
+ * right-hand side is formed in FP64 from the FP32 field and rounded once. */
 int mts_instance_create_ex(mts_ctx *ctx, const mts_setup *setup, const mts_physics_desc *physics,
                            int method, double dt, int precision, mts_instance **out);
 int mts_instance_precision(const mts_instance *inst, int *precision);

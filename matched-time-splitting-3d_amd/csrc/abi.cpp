@@ -170,9 +170,10 @@ struct mts_instance {
   mts::SideStream side;           // side_sh.s and its fork / join events (null: one stream)
   int precision = MTS_FP64;       // field storage and line solve
   DevBuf<double> u, r;            // solution (FP64 form); Douglas stage scratch
-  DevBuf<float> uf;               // solution (FP32 form)
+  DevBuf<float> uf, rf;           // the same in the FP32 form
   bool f32() const { return precision == MTS_FP32; }
   void *field() { return f32() ? static_cast<void *>(uf.p) : static_cast<void *>(u.p); }
+  void *stage() { return f32() ? static_cast<void *>(rf.p) : static_cast<void *>(r.p); }
   DevBuf<int8_t> ind;
   DevBuf<double> tab[3];
   DevBuf<int> blk_ptr[3];
@@ -224,7 +225,7 @@ struct mts_instance {
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
   size_t device_bytes() const {
-    size_t b = u.bytes() + uf.bytes() + r.bytes() + ind.bytes() + spec.bytes() + part_max.bytes() +
+    size_t b = u.bytes() + uf.bytes() + r.bytes() + rf.bytes() + ind.bytes() + spec.bytes() + part_max.bytes() +
                part_sq.bytes() + part_idx.bytes();
     for (int d = 0; d < 3; d++)
       b += tab[d].bytes() + blk_ptr[d].bytes() + blk_hdr[d].bytes() + blk_val[d].bytes() +
@@ -354,8 +355,6 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
   if (!setup || !physics) return fail(MTS_E_ARG, "null argument");
   if (method < MTS_ADI || method > MTS_TS) return fail(MTS_E_ARG, "unknown method");
   if (precision != MTS_FP64 && precision != MTS_FP32) return fail(MTS_E_ARG, "unknown precision");
-  if (precision == MTS_FP32 && method == MTS_ADI)
-    return fail(MTS_E_DISABLED, "the FP32 form covers LOD-IE, LOD-CN and TS; Douglas-ADI runs in FP64");
   if (!(dt > 0.0)) return fail(MTS_E_ARG, "dt must be positive");
   mts::BetaModel beta;
   std::string err;
@@ -457,7 +456,8 @@ int mts::plan_instance(const mts_setup *setup, const mts_physics_desc *physics, 
     }
     // exactly what instance_from_plan allocates
     size_t b = p->field_elems * ((precision == MTS_FP32 ? sizeof(float) : sizeof(double)) +
-                                 (method == MTS_ADI ? sizeof(double) : 0) + sizeof(int8_t));
+                                 (method == MTS_ADI ? (precision == MTS_FP32 ? sizeof(float) : sizeof(double)) : 0) +
+                                 sizeof(int8_t));
     for (int d = 0; d < 3; d++)
       b += t.tab[d].size() * sizeof(double) + t.blk_ptr[d].size() * sizeof(int) +
            t.blk_hdr[d].size() * sizeof(mts::BlockHdr) + t.blk_val[d].size() * sizeof(double) +
@@ -537,7 +537,10 @@ int mts::instance_from_plan(mts_ctx *ctx, mts::PlanPtr plan, mts_instance **out)
     MTS_HIP(in->u.alloc(in->field_elems, st));
     MTS_HIP(hipMemsetAsync(in->u.p, 0, in->field_elems * sizeof(double), st));
   }
-  if (p.method == MTS_ADI) {
+  if (p.method == MTS_ADI && in->f32()) {
+    MTS_HIP(in->rf.alloc(in->field_elems, st));
+    MTS_HIP(hipMemsetAsync(in->rf.p, 0, in->field_elems * sizeof(float), st));
+  } else if (p.method == MTS_ADI) {
     MTS_HIP(in->r.alloc(in->field_elems, st));
     MTS_HIP(hipMemsetAsync(in->r.p, 0, in->field_elems * sizeof(double), st));
   }
@@ -1040,8 +1043,8 @@ int mts_step(mts_instance *in, double t_next) {
     Rhs1Args r1;
     r1.g = in->geom;
     r1.P = in->phys;
-    r1.u = in->u.p;
-    r1.R = in->r.p;
+    r1.u = in->field();
+    r1.R = in->stage();
     r1.dt = dt;
     r1.e = e1;
     SpecArgs sp;
@@ -1050,40 +1053,41 @@ int mts_step(mts_instance *in, double t_next) {
     for (int d = 0; d < 3; d++) sp.mib[d] = in->mib[d];
     sp.spec = in->spec.p;
     sp.nspec = in->nspec;
-    sp.u = in->u.p;
-    sp.R = in->r.p;
+    sp.u = in->field();
+    sp.R = in->stage();
     sp.dt = dt;
     sp.e = e1;
     // Fused (the default): the x sweep computes the stage-1 right-hand side
     // of u^n as it stages its tiles (kFlagPre1), so only the MIB-affected
     // nodes are computed beforehand (into r, where the x sweep reads them) and
-    // R is never written out and read back.  MTS_DOUGLAS_FUSED=0: the whole
-    // right-hand side into r first, then the x sweep over r.
-    const bool fused = in->douglas_fused;
+    // R is never written out and read back.  MTS_DOUGLAS_FUSED=0, and the
+    // FP32 form: the whole right-hand side into r first, then the x sweep
+    // over r.
+    const bool fused = in->douglas_fused && !in->f32();
     MTS_HIP(timed(in, fused ? "adi_spec" : "adi_rhs1",
-                  [&] { return launch_rhs1(r1, sp, !fused, st); }));
+                  [&] { return launch_rhs1(r1, sp, !fused, in->f32(), st); }));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
     a.mib = in->mib[0];
-    a.A = fused ? in->u.p : in->r.p;
-    a.S = fused ? in->r.p : nullptr;
-    a.out = in->r.p;
+    a.A = fused ? in->field() : in->stage();
+    a.S = fused ? in->stage() : nullptr;
+    a.out = in->stage();
     a.in_analytic = fused ? 1 : 0;  // the x sweep's end rows: boundary values at t^{n+1}
     a.e_in = e1;
     MTS_HIP(timed(in, "adi_x", [&] {
       return line_launch(in, 0, kRhsCopy | kFlagSolve | (fused ? kFlagPre1 : 0), a, st);
     }));
     a.in_analytic = 0;
-    a.A = in->r.p;
+    a.A = in->stage();
     a.mib = in->mib[1];
-    a.S = in->u.p;
+    a.S = in->field();
     MTS_HIP(timed(in, "adi_y", [&] { return line_launch(in, 1, kRhsDouglas | kFlagSolve, a, st); }));
     a.mib = in->mib[2];
-    a.out = in->u.p;
+    a.out = in->field();
     MTS_HIP(timed(in, "adi_z", [&] { return line_launch(in, 2, kRhsDouglas | kFlagSolve, a, st); }));
     MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->u.p, false, e1, 0.0, st);
+      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, 0.0, st);
     }));
   } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
     const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;

@@ -356,9 +356,10 @@ double rhs1_plain(const Rhs1Args &p, int ix, int iy, int iz) {
   double vy[3], vz[3];
   op_weights(p.P, g, 1, out, ix, iy, iz, vy);
   op_weights(p.P, g, 2, out, ix, iy, iz, vz);
-  const double u = p.u[c];
-  double sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
-  double sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+  const double *pu = static_cast<const double *>(p.u);
+  const double u = pu[c];
+  double sum_y = pu[c - sy] * vy[0] + u * vy[1] + pu[c + sy] * vy[2];
+  double sum_z = pu[c - sz] * vz[0] + u * vz[1] + pu[c + sz] * vz[2];
   double r = p.dt * sum_y;
   r += p.dt * sum_z;
   r += u;

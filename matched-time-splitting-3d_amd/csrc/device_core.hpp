@@ -46,6 +46,18 @@ MTS_HD void ld2(const double *a, double (&v)[2]) {
   v[1] = a[1];
 #endif
 }
+// The same for two floats at an 8-byte aligned address (the FP32 field),
+// widened.
+MTS_HD void ld2(const float *a, double (&v)[2]) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const float2 t = *reinterpret_cast<const float2 *>(a);
+  v[0] = (double)t.x;
+  v[1] = (double)t.y;
+#else
+  v[0] = (double)a[0];
+  v[1] = (double)a[1];
+#endif
+}
 
 // ------------------------------------------------------------ geometry
 MTS_HD double tabv(const DevGeom &g, int a, int kind, int i) {
@@ -252,11 +264,12 @@ MTS_HD double src_jump(const DevMib &mib, int bidx, int rr, double e,
 }
 
 // fictitious-point stencil sum at block row rr, reading the field from global memory
-MTS_HD double block_sum(const DevMib &mib, int bidx, int rr,
-                                            const double *u, size_t base, size_t rs) {
+template <typename FT>
+MTS_HD double block_sum(const DevMib &mib, int bidx, int rr, const FT *u, size_t base,
+                        size_t rs) {
   const BlockHdr h = mib.hdr[bidx];
   const double *v = mib.val + h.off;
-  auto F = [&](int i) { return u[base + (size_t)i * rs]; };
+  auto F = [&](int i) { return (double)u[base + (size_t)i * rs]; };
   if (h.type == kBlkIrr) return rr == 0 ? irr_sum0(v, F, h.ll) : irr_sum1(v, F, h.ll);
   if (rr == 0) return cor_sum0(v, F, h.ll);
   if (rr == 1) return cor_sum1(v, F, h.ll, false, 0.0, 0.0);
@@ -288,15 +301,18 @@ MTS_HD double rhs1_value(const PhysConst &P, double dt, double e, bool out, cons
   r += dt * source_with(P, e, out, t);
   return r;
 }
+template <typename F = double>
 MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
   const DevGeom &g = p.g;
+  const F *pu = static_cast<const F *>(p.u);
+  F *pR = static_cast<F *>(p.R);
   const size_t c = node_index(g, ix, iy, iz);
   const bool bnd = ix == 0 || iy == 0 || iz == 0 || ix == g.n[0] - 1 || iy == g.n[1] - 1 ||
                    iz == g.n[2] - 1;
   const size_t sy = bnd ? 0 : (size_t)g.px, sz = bnd ? 0 : (size_t)g.n[1] * g.px;
   const signed char ind = g.ind[c];
-  const double u = p.u[c];
-  const double uym = p.u[c - sy], uyp = p.u[c + sy], uzm = p.u[c - sz], uzp = p.u[c + sz];
+  const double u = pu[c];
+  const double uym = pu[c - sy], uyp = pu[c + sy], uzm = pu[c - sz], uzp = pu[c + sz];
   Rhs1Trig t;
   t.s[0] = tabv(g, 0, kTabSin, ix);
   t.s[1] = tabv(g, 1, kTabSin, iy);
@@ -307,18 +323,21 @@ MTS_HD void rhs1_node(const Rhs1Args &p, int ix, int iy, int iz) {
   t.cym = tabv(g, 1, kTabCosM, iy);
   t.cyp = tabv(g, 1, kTabCosP, iy);
   if (bnd) {
-    p.R[c] = outer_with(p.e, t);
+    pR[c] = (F)outer_with(p.e, t);
     return;
   }
-  p.R[c] = rhs1_value(p.P, p.dt, p.e, ind > 0, t, u, uym, uyp, uzm, uzp);
+  pR[c] = (F)rhs1_value(p.P, p.dt, p.e, ind > 0, t, u, uym, uyp, uzm, uzp);
 }
 
 // The same for the node pair (2 pp, 2 pp + 1) of x row `row` (= iz * ny +
 // iy), pp < px / 2: one 16-byte load per field row, one 16-byte store when
 // both nodes are written.  Nodes flagged as MIB-affected (|ind| == 2, spec_node
 // writes them) and the x padding are left alone.
+template <typename F = double>
 MTS_HD void rhs1_pair(const Rhs1Args &p, unsigned row, unsigned pp) {
   const DevGeom &g = p.g;
+  const F *pu = static_cast<const F *>(p.u);
+  F *pR = static_cast<F *>(p.R);
   const int n0 = g.n[0], n1 = g.n[1], n2 = g.n[2];
   const int iy = (int)(row % (unsigned)n1), iz = (int)(row / (unsigned)n1);
   const int ix0 = 2 * (int)pp;
@@ -330,11 +349,11 @@ MTS_HD void rhs1_pair(const Rhs1Args &p, unsigned row, unsigned pp) {
   // (the pitch is even and a whole number of 16-byte units, so c is 16-byte
   // aligned; the second element of a pair past the row's end is padding)
   double u[2], uym[2], uyp[2], uzm[2], uzp[2];
-  ld2(p.u + c, u);
-  ld2(p.u + c - sy, uym);
-  ld2(p.u + c + sy, uyp);
-  ld2(p.u + c - sz, uzm);
-  ld2(p.u + c + sz, uzp);
+  ld2(pu + c, u);
+  ld2(pu + c - sy, uym);
+  ld2(pu + c + sy, uyp);
+  ld2(pu + c - sz, uzm);
+  ld2(pu + c + sz, uzp);
   const signed char i0 = g.ind[c], i1 = g.ind[c + 1];
   Rhs1Trig t;
   t.s[1] = tabv(g, 1, kTabSin, iy);
@@ -358,39 +377,41 @@ MTS_HD void rhs1_pair(const Rhs1Args &p, unsigned row, unsigned pp) {
                : rhs1_value(p.P, p.dt, p.e, ind > 0, t, u[q], uym[q], uyp[q], uzm[q], uzp[q]);
   }
   if (w[0] && w[1]) {
-    p.R[c] = r[0];
-    p.R[c + 1] = r[1];
+    pR[c] = (F)r[0];
+    pR[c + 1] = (F)r[1];
   } else {
-    if (w[0]) p.R[c] = r[0];
-    if (w[1]) p.R[c + 1] = r[1];
+    if (w[0]) pR[c] = (F)r[0];
+    if (w[1]) pR[c + 1] = (F)r[1];
   }
 }
 
 // Douglas stage-1 right-hand side at the t-th MIB-affected node: the
 // fictitious-point stencils along y and z and the source jump terms of all
 // three directions (douglas_adi.cc:116-312, Src_2nd :1158-1428).
+template <typename F = double>
 MTS_HD void spec_node(const SpecArgs &p, int t) {
   const DevGeom &g = p.g;
+  const F *pu = static_cast<const F *>(p.u);
   const int *rec = p.spec + (size_t)t * kSpecLen;
   const int ix = rec[0], iy = rec[1], iz = rec[2];
   const size_t c = node_index(g, ix, iy, iz);
   const bool out = g.ind[c] > 0;
   const size_t sy = (size_t)g.px, sz = (size_t)g.n[1] * g.px;
-  const double u = p.u[c];
+  const double u = pu[c];
   double sum_y, sum_z;
   if (rec[3 + 1] >= 0) {
-    sum_y = block_sum(p.mib[1], rec[3 + 1], rec[6 + 1], p.u, c - (size_t)iy * sy, sy);
+    sum_y = block_sum(p.mib[1], rec[3 + 1], rec[6 + 1], pu, c - (size_t)iy * sy, sy);
   } else {
     double vy[3];
     op_weights(p.P, g, 1, out, ix, iy, iz, vy);
-    sum_y = p.u[c - sy] * vy[0] + u * vy[1] + p.u[c + sy] * vy[2];
+    sum_y = (double)pu[c - sy] * vy[0] + u * vy[1] + (double)pu[c + sy] * vy[2];
   }
   if (rec[3 + 2] >= 0) {
-    sum_z = block_sum(p.mib[2], rec[3 + 2], rec[6 + 2], p.u, c - (size_t)iz * sz, sz);
+    sum_z = block_sum(p.mib[2], rec[3 + 2], rec[6 + 2], pu, c - (size_t)iz * sz, sz);
   } else {
     double vz[3];
     op_weights(p.P, g, 2, out, ix, iy, iz, vz);
-    sum_z = p.u[c - sz] * vz[0] + u * vz[1] + p.u[c + sz] * vz[2];
+    sum_z = (double)pu[c - sz] * vz[0] + u * vz[1] + (double)pu[c + sz] * vz[2];
   }
   double src = source(p.P, g, p.e, out, ix, iy, iz);
   for (int d = 0; d < 3; d++)
@@ -399,7 +420,7 @@ MTS_HD void spec_node(const SpecArgs &p, int t) {
   r += p.dt * sum_z;
   r += u;
   r += p.dt * src;
-  p.R[c] = r;
+  static_cast<F *>(p.R)[c] = (F)r;
 }
 
 // ------------------------------------------------------- line stages

@@ -177,8 +177,8 @@ struct LineArgs {
 struct Rhs1Args {
   DevGeom g;
   PhysConst P;
-  const double *u;
-  double *R;
+  const void *u;  // the field: double, or float in the FP32 form
+  void *R;        // the stage-1 buffer, of the field's type
   double dt;
   double e;  // exp(-t^{n+1})
 };
@@ -201,8 +201,8 @@ struct SpecArgs {
   DevMib mib[3];
   const int *spec;
   int nspec;
-  const double *u;
-  double *R;
+  const void *u;  // of the field's type (as Rhs1Args)
+  void *R;
   double dt;
   double e;
 };

@@ -167,11 +167,12 @@ __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int
 // and the y and z neighbours a block reads were fetched into that XCD's L2 by
 // the blocks just before it.
 constexpr int kXcds = 8;
+template <typename F>
 __global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsigned npairs,
                                                    unsigned nper, unsigned hp, double inv_hp) {
   if (blockIdx.x >= nper * kXcds) {
     const int t = (int)(blockIdx.x - nper * kXcds) * 256 + (int)threadIdx.x;
-    if (t < sp.nspec) spec_node(sp, t);
+    if (t < sp.nspec) spec_node<F>(sp, t);
     return;
   }
   const unsigned b = (blockIdx.x % kXcds) * nper + blockIdx.x / kXcds;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsi
   unsigned row = (unsigned)((double)t * inv_hp);
   if (row * hp > t) row--;
   else if ((row + 1) * hp <= t) row++;
-  rhs1_pair(p, row, t - row * hp);
+  rhs1_pair<F>(p, row, t - row * hp);
 }
 
 // ------------------------------------------------------- boundary faces
@@ -481,7 +482,8 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
   return one(args, args.min_lanes, s);
 }
 
-hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, hipStream_t s) {
+hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, bool f32,
+                       hipStream_t s) {
   if (a.g.px % 2 != 0 || sp.nspec < 0) return hipErrorInvalidValue;
   const unsigned hp = (unsigned)(a.g.px / 2);
   const size_t pairs = pairs_too ? (size_t)hp * a.g.n[1] * a.g.n[2] : 0;
@@ -489,8 +491,13 @@ hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, hi
   const unsigned nper = (unsigned)(((pairs + 255) / 256 + kXcds - 1) / kXcds);
   const unsigned nspec_blocks = (unsigned)((sp.nspec + 255) / 256);
   if (nper * kXcds + nspec_blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(rhs1_kernel, dim3(nper * kXcds + nspec_blocks), dim3(256), 0, s, a, sp,
-                     (unsigned)pairs, nper, hp, 1.0 / hp);
+  const dim3 grid(nper * kXcds + nspec_blocks);
+  if (f32)
+    hipLaunchKernelGGL(rhs1_kernel<float>, grid, dim3(256), 0, s, a, sp, (unsigned)pairs, nper,
+                       hp, 1.0 / hp);
+  else
+    hipLaunchKernelGGL(rhs1_kernel<double>, grid, dim3(256), 0, s, a, sp, (unsigned)pairs, nper,
+                       hp, 1.0 / hp);
   return hipGetLastError();
 }
 

@@ -38,7 +38,8 @@ MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
 // stencil on every node but the MIB-affected ones (classification +-2), and
 // those from sp (spec_node), in one launch; pairs_too false: the
 // MIB-affected nodes only (the x sweep computes the rest, kFlagPre1).
-hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, hipStream_t s);
+hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, bool f32,
+                       hipStream_t s);
 
 // Boundary faces: out = exact outer solution at e (+ cf * f(e) when cf != 0);
 // f32: out is an FP32 field.

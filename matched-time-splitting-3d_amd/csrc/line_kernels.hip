@@ -822,8 +822,10 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
 
 hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream_t s) {
   if (f32) {
-    // the FP32 form: LOD and TS stages (Douglas-ADI runs in FP64 only)
+    // the FP32 form: LOD, TS and Douglas-ADI (its stage 1 outside the x sweep)
     switch (flags) {
+      MTS_LINE_CASE(kRhsCopy | kFlagSolve, float)
+      MTS_LINE_CASE(kRhsDouglas | kFlagSolve, float)
       MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
       MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
       MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
