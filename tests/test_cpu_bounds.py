@@ -25,6 +25,10 @@ GEOMS = {
     "wide_x": ["600x20x18", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
     "wide_y": ["18x300x22", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
     "wide_z": ["20x18x1024", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
+    # 1024-node x and y lines: 64-lane groups, the two LDS tiles of the
+    # explicit stages above 64 KiB
+    "wide_x1024": ["1024x18x20", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
+    "wide_y1024": ["18x1024x20", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
     # full splits along x: idle lanes past the last segment; a short last segment
     "full_idle": ["54x20x22", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],
     "full_short": ["255x18x20", "ellipsoid", "0.7", "0.5", "0.4", E1, "sincos", "10", "1"],

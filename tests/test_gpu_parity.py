@@ -129,11 +129,12 @@ def test_gpu_matches_oracle_per_step(mts, oracle, ctx, method, mib, ci):
         u = ug
 
 
-# x / y / z lines over 64-, 32- and 16-lane groups; (54, 20, 22) and
+# x / y / z lines over 64-, 32- and 16-lane groups (1024-node x and y lines:
+# the explicit stages' two LDS tiles above 64 KiB); (54, 20, 22) and
 # (255, 18, 20): full splits along x with idle lanes past the last segment and
 # with a last segment 3 rows short of the register bound
 WIDE = [(600, 20, 18), (18, 300, 22), (20, 18, 1024), (130, 66, 34), (54, 20, 22),
-        (255, 18, 20)]
+        (255, 18, 20), (1024, 18, 20), (18, 1024, 20)]
 
 
 @pytest.mark.parametrize("method", ["lod-ie", "lod-cn", "ts"])
