@@ -388,6 +388,15 @@ def bench_batch(args):
     same = True
     for inst, (steps, t_end), le, te in zip(insts, out, loop_err, ends):
         same &= t_end == te and inst.error(t_end)[0] == le
+    # the same in lockstep, one launch per line sweep across the solves
+    for inst in insts:
+        inst.initialize(0.0)
+        inst.sync()
+    t0 = time.perf_counter()
+    outm = mts.batch_run(insts, 0.0, [d["T"] for d in ds], multi=True)
+    multi_s = time.perf_counter() - t0
+    for inst, (steps, t_end), le, te in zip(insts, outm, loop_err, ends):
+        same &= t_end == te and inst.error(t_end)[0] == le
     # the same solves batched in the FP32 form
     f32 = None
     if args.precision == "fp32":
@@ -410,6 +419,12 @@ def bench_batch(args):
         t0 = time.perf_counter()
         mts.batch_run(insts32, 0.0, [d["T"] for d in ds], graph=True)
         f32g_s = time.perf_counter() - t0
+        for inst in insts32:
+            inst.initialize(0.0)
+            inst.sync()
+        t0 = time.perf_counter()
+        mts.batch_run(insts32, 0.0, [d["T"] for d in ds], multi=True)
+        f32m_s = time.perf_counter() - t0
         ratios = []
         for inst, (steps, t_end), le in zip(insts32, out32, loop_err):
             l32 = inst.error(t_end)[0]
@@ -420,6 +435,8 @@ def bench_batch(args):
                "speedup_vs_fp64_batched": batch_s / f32_s,
                "graph_batched_s": f32g_s, "graph_value": work / f32g_s,
                "graph_speedup_vs_fp64_loop": loop_s / f32g_s,
+               "multi_batched_s": f32m_s, "multi_value": work / f32m_s,
+               "multi_speedup_vs_fp64_loop": loop_s / f32m_s,
                "linf_ratio_max": max(ratios) if ratios else None,
                "linf_ratio_median": float(np.median(ratios)) if ratios else None,
                "solves_compared": len(ratios)}
@@ -443,6 +460,9 @@ def bench_batch(args):
         "one_after_another_s": loop_s,
         "one_after_another_value": work / loop_s,
         "speedup": loop_s / batch_s,
+        "multi_batched_s": multi_s,
+        "multi_value": work / multi_s,
+        "multi_speedup": loop_s / multi_s,
         "same_linf_batched_and_alone": bool(same),
         "max_linf": max(loop_err) if loop_err else None,
         "fp32_form": f32,

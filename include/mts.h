@@ -300,6 +300,19 @@ int mts_instance_device_bytes(const mts_instance *inst, size_t *bytes);
 int mts_batch_run(mts_instance *const *insts, int count, const double *t0, const double *t_finish,
                   int *steps, double *t_end);
 
+/* mts_batch_run with one launch per line sweep across instances: the
+ * instances of one method and precision step in lockstep (each with its own
+ * dt and t; those with fewer steps drop out), and each solve sweep of a step
+ * runs as one launch over the tiles of all of them (one per group of
+ * instances whose line split and register bound match), on the first
+ * instance's stream; the other launches of a step (Douglas stage 1, the
+ * boundary faces, TS's explicit sweeps) follow per instance on that stream.
+ * Groups of other methods or precisions run one after another.  The same
+ * kernels' arithmetic as mts_run, bit for bit.  No counterpart in the
+ * reference. */
+int mts_batch_run_multi(mts_instance *const *insts, int count, const double *t0,
+                        const double *t_finish, int *steps, double *t_end);
+
 /* mts_batch_run with each instance's time loop as one graph (mts_run_graph),
  * the graphs launched back to back on the instances' streams. */
 int mts_batch_run_graph(mts_instance *const *insts, int count, const double *t0,

@@ -157,6 +157,7 @@ def lib() -> C.CDLL:
             "mts_instance_device_bytes": ([VP, P(C.c_size_t)], I),
             "mts_batch_run": ([VP, I, VP, VP, VP, VP], I),
             "mts_batch_run_graph": ([VP, I, VP, VP, VP, VP], I),
+            "mts_batch_run_multi": ([VP, I, VP, VP, VP, VP], I),
             "mts_run_graph": ([VP, D, D, P(I), P(D)], I),
             "mts_server_create": ([VP, C.c_size_t, I, I, P(VP)], I),
             "mts_server_submit": ([VP, P(Request), P(C.c_longlong)], I),
@@ -460,17 +461,19 @@ def version() -> str:
     return lib().mts_version().decode()
 
 
-def batch_run(instances, t0, t_finish, graph: bool = False):
+def batch_run(instances, t0, t_finish, graph: bool = False, multi: bool = False):
     """Runs the time loops of several instances with their steps interleaved
-    (``mts_batch_run``), or each loop as one graph (``mts_batch_run_graph``);
-    returns [(steps, t_end)] per instance."""
+    (``mts_batch_run``), each loop as one graph (``mts_batch_run_graph``), or
+    in lockstep with one launch per line sweep across instances
+    (``mts_batch_run_multi``); returns [(steps, t_end)] per instance."""
     n = len(instances)
     t0 = np.ascontiguousarray(np.broadcast_to(np.asarray(t0, np.float64), (n,)))
     tf = np.ascontiguousarray(np.broadcast_to(np.asarray(t_finish, np.float64), (n,)))
     hs = (C.c_void_p * n)(*[i._h.value for i in instances])
     steps = np.zeros(n, np.int32)
     t_end = np.zeros(n, np.float64)
-    f = lib().mts_batch_run_graph if graph else lib().mts_batch_run
+    f = (lib().mts_batch_run_multi if multi else
+         lib().mts_batch_run_graph if graph else lib().mts_batch_run)
     _check(f(hs, n, _dptr(t0), _dptr(tf), _dptr(steps), _dptr(t_end)))
     return [(int(steps[i]), float(t_end[i])) for i in range(n)]
 

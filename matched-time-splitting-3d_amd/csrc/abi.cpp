@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -205,6 +206,16 @@ struct mts_instance {
   mts::DevMib mib[3];
   // per-stage event timing
   bool profile = false;
+  // Recording a step instead of launching it (mts_batch_run_multi): solve
+  // line stages as their arguments, every other launch as a call on a given
+  // stream.
+  struct StepOp {
+    int line = 0;  // 1: a split solve stage (dir, flags, a)
+    int dir = 0, flags = 0;
+    mts::LineArgs a;
+    std::function<hipError_t(hipStream_t)> fn;
+  };
+  std::vector<StepOp> *rec = nullptr;
   std::vector<hipEvent_t> ev_pool;
   size_t ev_used = 0;
   std::vector<std::pair<int, size_t>> pending;  // stage id, first event index
@@ -1029,7 +1040,36 @@ static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::L
     a.tiles[c] = solve ? in->tiles[dir][c].p : nullptr;
     a.ntiles[c] = solve ? (int)in->tiles[dir][c].n : 0;
   }
+  if (in->rec) {
+    mts_instance::StepOp op;
+    op.dir = dir;
+    op.flags = flags;
+    op.a = a;
+    if (solve) {
+      op.line = 1;
+    } else {
+      const bool f32 = in->f32();
+      op.fn = [dir, flags, f32, a](hipStream_t s) { return mts::launch_line(dir, flags, f32, a, s); };
+    }
+    in->rec->push_back(std::move(op));
+    return hipSuccess;
+  }
   return mts::launch_line(dir, flags, in->f32(), a, st, &in->side);
+}
+
+// A launch other than a line stage: run on the instance's stream (timed when
+// profiling), or recorded.
+extern "C++" {
+template <class F>
+static hipError_t other_op(mts_instance *in, const char *name, F fn) {
+  if (in->rec) {
+    mts_instance::StepOp op;
+    op.fn = fn;
+    in->rec->push_back(std::move(op));
+    return hipSuccess;
+  }
+  return timed(in, name, [&] { return fn(in->stream); });
+}
 }
 
 int mts_step(mts_instance *in, double t_next) {
@@ -1064,8 +1104,10 @@ int mts_step(mts_instance *in, double t_next) {
     // FP32 form: the whole right-hand side into r first, then the x sweep
     // over r.
     const bool fused = in->douglas_fused && !in->f32();
-    MTS_HIP(timed(in, fused ? "adi_spec" : "adi_rhs1",
-                  [&] { return launch_rhs1(r1, sp, !fused, in->f32(), st); }));
+    const bool f32 = in->f32();
+    MTS_HIP(other_op(in, fused ? "adi_spec" : "adi_rhs1", [=](hipStream_t s) {
+      return launch_rhs1(r1, sp, !fused, f32, s);
+    }));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
@@ -1086,8 +1128,11 @@ int mts_step(mts_instance *in, double t_next) {
     a.mib = in->mib[2];
     a.out = in->field();
     MTS_HIP(timed(in, "adi_z", [&] { return line_launch(in, 2, kRhsDouglas | kFlagSolve, a, st); }));
-    MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, 0.0, st);
+    const DevGeom geom = in->geom;
+    const PhysConst phys = in->phys;
+    void *field = in->field();
+    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
+      return launch_faces(geom, phys, field, f32, e1, 0.0, s);
     }));
   } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
     const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;
@@ -1108,8 +1153,12 @@ int mts_step(mts_instance *in, double t_next) {
     MTS_HIP(timed(in, "lod_z", [&] {
       return line_launch(in, 2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
     }));
-    MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, dt, st);
+    const DevGeom geom = in->geom;
+    const PhysConst phys = in->phys;
+    void *field = in->field();
+    const bool f32 = in->f32();
+    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
+      return launch_faces(geom, phys, field, f32, e1, dt, s);
     }));
   } else {
     const double en = std::exp(-(t_next - dt));
@@ -1140,9 +1189,114 @@ int mts_step(mts_instance *in, double t_next) {
     MTS_HIP(timed(in, "ts_iy", [&] { return line_launch(in, 1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[0];
     MTS_HIP(timed(in, "ts_ix", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
-    MTS_HIP(timed(in, "faces", [&] {
-      return launch_faces(in->geom, in->phys, in->field(), in->f32(), e1, 0.0, st);
+    const DevGeom geom = in->geom;
+    const PhysConst phys = in->phys;
+    void *field = in->field();
+    const bool f32 = in->f32();
+    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
+      return launch_faces(geom, phys, field, f32, e1, 0.0, s);
     }));
+  }
+  return MTS_OK;
+}
+
+int mts_batch_run_multi(mts_instance *const *insts, int count, const double *t0,
+                        const double *t_finish, int *steps, double *t_end) {
+  if (!insts || count < 1 || !t0 || !t_finish) return fail(MTS_E_ARG, "null argument");
+  for (int i = 0; i < count; i++) {
+    if (!insts[i]) return fail(MTS_E_ARG, "null instance");
+    if (insts[i]->ctx != insts[0]->ctx) return fail(MTS_E_ARG, "instances of different contexts");
+  }
+  MTS_HIP(hipSetDevice(insts[0]->ctx->device));
+  // every instance's earlier work is done (their buffers are then usable on
+  // the batch stream)
+  for (int i = 0; i < count; i++) MTS_HIP(hipStreamSynchronize(insts[i]->stream));
+  const hipStream_t s = insts[0]->stream;
+  std::vector<int> loops(count);
+  std::vector<double> t(count);
+  for (int i = 0; i < count; i++) {
+    loops[i] = (int)(t_finish[i] / insts[i]->dt);
+    t[i] = t0[i];
+  }
+  // groups of one method and precision, in order of first appearance
+  std::vector<std::vector<int>> groups;
+  for (int i = 0; i < count; i++) {
+    size_t g = 0;
+    while (g < groups.size() && !(insts[groups[g][0]]->method == insts[i]->method &&
+                                  insts[groups[g][0]]->precision == insts[i]->precision))
+      g++;
+    if (g == groups.size()) groups.emplace_back();
+    groups[g].push_back(i);
+  }
+  mts::MultiBuf buf;
+  std::vector<std::vector<mts_instance::StepOp>> ops(count);
+  std::vector<mts::LineArgs> args;
+  int rc = MTS_OK;
+  for (const auto &grp : groups) {
+    int maxl = 0;
+    for (int i : grp) maxl = std::max(maxl, loops[i]);
+    for (int k = 0; k < maxl && rc == MTS_OK; k++) {
+      std::vector<int> act;
+      for (int i : grp)
+        if (k < loops[i]) act.push_back(i);
+      for (int i : act) {
+        mts_instance *in = insts[i];
+        ops[i].clear();
+        const bool prof = in->profile;
+        in->profile = false;
+        in->rec = &ops[i];
+        t[i] += in->dt;
+        rc = mts_step(in, t[i]);
+        in->rec = nullptr;
+        in->profile = prof;
+        if (rc) break;
+      }
+      if (rc) break;
+      const size_t nops = ops[act[0]].size();
+      for (int i : act)
+        if (ops[i].size() != nops) {
+          rc = fail(MTS_E_ARG, "instances of one method recorded different steps");
+          break;
+        }
+      for (size_t o = 0; o < nops && rc == MTS_OK; o++) {
+        const mts_instance::StepOp &first = ops[act[0]][o];
+        if (first.line) {
+          args.clear();
+          for (int i : act) {
+            const mts_instance::StepOp &op = ops[i][o];
+            if (!op.line || op.dir != first.dir || op.flags != first.flags) {
+              rc = fail(MTS_E_ARG, "instances of one method recorded different steps");
+              break;
+            }
+            args.push_back(op.a);
+          }
+          if (rc) break;
+          const hipError_t e = mts::launch_line_multi(first.dir, first.flags, insts[act[0]]->f32(),
+                                                      args.data(), (int)args.size(), buf, s);
+          if (e != hipSuccess) rc = fail(MTS_E_DEVICE, hipGetErrorString(e));
+        } else {
+          for (int i : act) {
+            if (ops[i][o].line) {
+              rc = fail(MTS_E_ARG, "instances of one method recorded different steps");
+              break;
+            }
+            const hipError_t e = ops[i][o].fn(s);
+            if (e != hipSuccess) {
+              rc = fail(MTS_E_DEVICE, hipGetErrorString(e));
+              break;
+            }
+          }
+        }
+      }
+    }
+    if (rc) break;
+  }
+  const hipError_t e = hipStreamSynchronize(s);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(MTS_E_DEVICE, hipGetErrorString(e));
+  for (int i = 0; i < count; i++) {
+    if (steps) steps[i] = loops[i];
+    if (t_end) t_end[i] = t[i];
   }
   return MTS_OK;
 }

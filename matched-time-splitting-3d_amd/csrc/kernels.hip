@@ -23,6 +23,7 @@
 // disabled for this file) except the tridiagonal elimination order.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
 #include <mutex>
 #include <vector>
 
@@ -480,6 +481,120 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
     return e;
   }
   return one(args, args.min_lanes, s);
+}
+
+MultiBuf::~MultiBuf() {
+  for (int i = 0; i < 2; i++) {
+    if (copied[i]) (void)hipEventSynchronize(copied[i]);
+    if (copied[i]) (void)hipEventDestroy(copied[i]);
+    if (host[i]) (void)hipHostFree(host[i]);
+  }
+  if (dev) (void)hipFree(dev);
+}
+
+hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args, int count,
+                             MultiBuf &buf, hipStream_t s) {
+  if (!(flags & kFlagSolve) || dir < 0 || dir > 2) return hipErrorInvalidValue;
+  struct Key {
+    int L, mmax, full, seq, regular;
+    bool operator==(const Key &o) const {
+      return L == o.L && mmax == o.mmax && full == o.full && seq == o.seq && regular == o.regular;
+    }
+  };
+  struct Group {
+    Key k;
+    std::vector<LineArgs> args;
+    std::vector<int2> work;
+  };
+  std::vector<Group> groups;
+  for (int i = 0; i < count; i++) {
+    const LineArgs &in = args[i];
+    if (!in.split) return hipErrorInvalidValue;
+    for (int regular = 0; regular < 2; regular++) {
+      const int nt = in.ntiles[regular ? 0 : 1];
+      if (nt <= 0) continue;
+      LineArgs a = in;
+      a.ntiles[regular ? 1 : 0] = 0;
+      const int lanes = regular ? in.min_lanes
+                                : (in.gen_lanes > in.min_lanes ? in.gen_lanes : in.min_lanes);
+      if (!make_line_part(a.g.n[dir], a.q, lanes)) return hipErrorInvalidValue;
+      if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads)
+        return hipErrorInvalidValue;
+      Key k{a.q.L, a.q.mmax, a.q.m == a.q.mmax ? 1 : 0, a.mib.seq != nullptr ? 1 : 0, regular};
+      size_t gi = 0;
+      while (gi < groups.size() && !(groups[gi].k == k)) gi++;
+      if (gi == groups.size()) groups.push_back(Group{k, {}, {}});
+      Group &g = groups[gi];
+      const int slot = (int)g.args.size();
+      g.args.push_back(a);
+      for (int t = 0; t < nt; t++) g.work.push_back(make_int2(slot, t));
+    }
+  }
+  if (groups.empty()) return hipSuccess;
+  // one staging image: per group its arguments, then its work entries
+  auto up16 = [](size_t b) { return (b + 15) / 16 * 16; };
+  std::vector<size_t> off_args(groups.size()), off_work(groups.size());
+  size_t bytes = 0;
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    off_args[gi] = bytes;
+    bytes = up16(bytes + groups[gi].args.size() * sizeof(LineArgs));
+    off_work[gi] = bytes;
+    bytes = up16(bytes + groups[gi].work.size() * sizeof(int2));
+  }
+  hipError_t e = hipSuccess;
+  if (bytes > buf.host_cap || bytes > buf.dev_cap) {
+    e = hipStreamSynchronize(s);  // (nothing in flight reads the old buffers)
+    if (e != hipSuccess) return e;
+    const size_t cap = bytes * 2;
+    for (int i = 0; i < 2; i++) {
+      if (buf.host[i]) (void)hipHostFree(buf.host[i]);
+      buf.host[i] = nullptr;
+      e = hipHostMalloc(&buf.host[i], cap, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+      if (!buf.copied[i]) {
+        e = hipEventCreateWithFlags(&buf.copied[i], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+      }
+    }
+    if (buf.dev) (void)hipFree(buf.dev);
+    buf.dev = nullptr;
+    e = hipMalloc(&buf.dev, cap);
+    if (e != hipSuccess) return e;
+    buf.host_cap = buf.dev_cap = cap;
+  }
+  // the slot's last copy has left it
+  const int sl = buf.slot;
+  buf.slot ^= 1;
+  e = hipEventSynchronize(buf.copied[sl]);
+  if (e != hipSuccess) return e;
+  char *h = static_cast<char *>(buf.host[sl]);
+  for (size_t gi = 0; gi < groups.size(); gi++) {
+    std::memcpy(h + off_args[gi], groups[gi].args.data(), groups[gi].args.size() * sizeof(LineArgs));
+    std::memcpy(h + off_work[gi], groups[gi].work.data(), groups[gi].work.size() * sizeof(int2));
+  }
+  e = hipMemcpyAsync(buf.dev, h, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(buf.copied[sl], s);
+  if (e != hipSuccess) return e;
+  char *d = static_cast<char *>(buf.dev);
+  for (size_t gi = 0; gi < groups.size() && e == hipSuccess; gi++) {
+    const Group &g = groups[gi];
+    const LineArgs *da = reinterpret_cast<const LineArgs *>(d + off_args[gi]);
+    const int2 *dw = reinterpret_cast<const int2 *>(d + off_work[gi]);
+    const int na = (int)g.args.size(), nw = (int)g.work.size();
+    const bool full = g.k.full != 0, seq = g.k.seq != 0, reg = g.k.regular != 0;
+#define MTS_ML(D, LL)                                                                          \
+  if (dir == D && g.k.L == LL) {                                                               \
+    e = launch_multi_dl_##D##_##LL(flags, seq, f32, g.k.mmax, full, reg, g.args.data(), na, da, \
+                                   dw, nw, s);                                                 \
+    continue;                                                                                  \
+  }
+    MTS_ML(0, 16) MTS_ML(0, 32) MTS_ML(0, 64)
+    MTS_ML(1, 16) MTS_ML(1, 32) MTS_ML(1, 64)
+    MTS_ML(2, 16) MTS_ML(2, 32) MTS_ML(2, 64)
+#undef MTS_ML
+    e = hipErrorInvalidValue;
+  }
+  return e;
 }
 
 hipError_t launch_rhs1(const Rhs1Args &a, const SpecArgs &sp, bool pairs_too, bool f32,

@@ -33,6 +33,39 @@ MTS_DECL_DL(0, 16) MTS_DECL_DL(0, 32) MTS_DECL_DL(0, 64)
 MTS_DECL_DL(1, 16) MTS_DECL_DL(1, 32) MTS_DECL_DL(1, 64)
 MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
 #undef MTS_DECL_DL
+#define MTS_DECL_MULTI(D, LL)                                                                 \
+  hipError_t launch_multi_dl_##D##_##LL(int flags, bool seq, bool f32, int mmax, bool full,     \
+                                        bool regular, const LineArgs *hargs, int nargs,         \
+                                        const LineArgs *dargs, const int2 *dwork, int nwork,    \
+                                        hipStream_t s);
+MTS_DECL_MULTI(0, 16) MTS_DECL_MULTI(0, 32) MTS_DECL_MULTI(0, 64)
+MTS_DECL_MULTI(1, 16) MTS_DECL_MULTI(1, 32) MTS_DECL_MULTI(1, 64)
+MTS_DECL_MULTI(2, 16) MTS_DECL_MULTI(2, 32) MTS_DECL_MULTI(2, 64)
+#undef MTS_DECL_MULTI
+
+// Staging of a multi-instance line stage's arguments and work lists: pinned
+// host memory (two slots, each reused once the copy out of it is done) and
+// one device buffer, grown as needed.  Used on one stream.
+struct MultiBuf {
+  void *host[2] = {nullptr, nullptr};
+  hipEvent_t copied[2] = {nullptr, nullptr};
+  size_t host_cap = 0;
+  int slot = 0;
+  void *dev = nullptr;
+  size_t dev_cap = 0;
+  MultiBuf() = default;
+  MultiBuf(const MultiBuf &) = delete;
+  MultiBuf &operator=(const MultiBuf &) = delete;
+  ~MultiBuf();
+};
+// One solve stage (dir, flags) of count instances in as few launches as
+// their kernels allow: the general and the regular tiles of every instance
+// whose split, register bound, full split and sequential lines match go into
+// one launch over all their tiles (line_kernel_multi / line_kernel_reg_multi).
+// args[i] are the instances' stage arguments as launch_line takes them (a
+// split solve stage).  Same arithmetic as launch_line per instance.
+hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args, int count,
+                             MultiBuf &buf, hipStream_t s);
 
 // Douglas stage 1 right-hand side (see device_core.hpp): the regular
 // stencil on every node but the MIB-affected ones (classification +-2), and

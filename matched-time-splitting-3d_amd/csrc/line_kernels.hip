@@ -568,6 +568,17 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   tile_coords<DIR, tile_lines(L), R>(p.split ? p.tiles[1] : nullptr, bx, by);
   line_body<DIR, L, MMAX, FLAGS, R>(p, bx, by);
 }
+// The general tiles of several instances' line stage in one launch:
+// workgroup b solves tile work[b].y of the list of instance work[b].x, whose
+// stage arguments are args[work[b].x] (in device memory).
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void line_kernel_multi(
+    const LineArgs *args, const int2 *work) {
+  const int2 w = work[blockIdx.x];
+  const LineArgs &p = args[w.x];
+  const int e = p.tiles[1][w.y];
+  line_body<DIR, L, MMAX, FLAGS, R>(p, tile_bx(e), tile_k(e) - 1);
+}
 
 // What one lane needs of one tile besides the field: the tile's place, the
 // lane's line (interior or not, side), its boundary values and the line's
@@ -613,7 +624,7 @@ __device__ __forceinline__ RegTileMeta<DIR> reg_tile_meta(const LineArgs &p, int
 // than line_kernel and four of its workgroups share a CU: while some load
 // their tile, others solve or store theirs.
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void line_kernel_reg(LineArgs p) {
+__device__ __forceinline__ void reg_body(const LineArgs &p, int tile) {
   using V2 = typename Pair2<R>::type;
   constexpr int RHS = FLAGS & 3;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
@@ -641,7 +652,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
   // stages read sin z / cos z from the axis tables, not an LDS copy), the
   // solve table of the sides present, then the field, all loads in flight
   // together
-  const RegTileMeta<DIR> cur = reg_tile_meta<DIR, L>(p, p.tiles[0][blockIdx.x], nullptr);
+  const RegTileMeta<DIR> cur = reg_tile_meta<DIR, L>(p, tile, nullptr);
   {
     // the solve tables straight into LDS (16-byte LDS-DMA loads: lane i of
     // a wavefront writes 16 bytes at the instruction's LDS base + 16 i); the
@@ -751,6 +762,19 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
     }
   }
 }
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void line_kernel_reg(LineArgs p) {
+  reg_body<DIR, L, MMAX, FLAGS, R>(p, p.tiles[0][blockIdx.x]);
+}
+// The regular tiles of several instances' line stage in one launch (as
+// line_kernel_multi).
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void line_kernel_reg_multi(
+    const LineArgs *args, const int2 *work) {
+  const int2 w = work[blockIdx.x];
+  const LineArgs &p = args[w.x];
+  reg_body<DIR, L, MMAX, FLAGS, R>(p, p.tiles[0][w.y]);
+}
 // A solve stage with tile lists runs the regular tiles with line_kernel_reg
 // and the others with line_kernel; any other stage runs line_kernel over the
 // whole grid.
@@ -789,6 +813,50 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+// One group of a multi-instance line stage: the general tiles (regular
+// false) or the regular tiles of the instances whose stage arguments are
+// hargs[0..nargs) (host copies, for the LDS size) and dargs (device), one
+// workgroup per entry of dwork[0..nwork).
+template <int DIR, int L, int MMAX, int FLAGS, typename R>
+hipError_t launch_multi_t(bool regular, const LineArgs *hargs, int nargs, const LineArgs *dargs,
+                          const int2 *dwork, int nwork, hipStream_t st) {
+  using LL = LdsLayout<DIR, L, MMAX, FLAGS, R>;
+  constexpr bool FULL = (FLAGS & kFlagFull) != 0;
+  size_t shm = 0;
+  for (int i = 0; i < nargs; i++) {
+    const LineArgs &a = hargs[i];
+    if (regular && !a.reg[FULL ? 1 : 0]) return hipErrorInvalidValue;
+    const size_t b = regular ? LL::reg_bytes(a.q.lds_len, a.reg_side, a.reg_sides)
+                             : LL::bytes(a.q.lds_len, a.g.n[2], false);
+    shm = b > shm ? b : shm;
+  }
+  if (nwork <= 0) return hipSuccess;
+  if (regular)
+    hipLaunchKernelGGL((line_kernel_reg_multi<DIR, L, MMAX, FLAGS, R>), dim3(nwork), dim3(kBlock),
+                       shm, st, dargs, dwork);
+  else
+    hipLaunchKernelGGL((line_kernel_multi<DIR, L, MMAX, FLAGS, R>), dim3(nwork), dim3(kBlock), shm,
+                       st, dargs, dwork);
+  return hipGetLastError();
+}
+template <int FLAGS, typename R>
+hipError_t multi_flags(int mmax, bool full, bool regular, const LineArgs *hargs, int nargs,
+                       const LineArgs *dargs, const int2 *dwork, int nwork, hipStream_t st) {
+  constexpr int D = MTS_LINE_DIR, LL = MTS_LINE_L;
+  constexpr bool CAN_FULL = (FLAGS & kFlagSolve) && !(FLAGS & kFlagSeq);
+  constexpr int FF = CAN_FULL ? (FLAGS | kFlagFull) : FLAGS;
+  full = CAN_FULL && full;
+#define MTS_MULTI_M(M)                                                                         \
+  if (mmax == M)                                                                               \
+    return full ? launch_multi_t<D, LL, M, FF, R>(regular, hargs, nargs, dargs, dwork, nwork, st) \
+                : launch_multi_t<D, LL, M, FLAGS, R>(regular, hargs, nargs, dargs, dwork, nwork, st);
+  MTS_MULTI_M(4)
+  MTS_MULTI_M(8)
+  MTS_MULTI_M(16)
+#undef MTS_MULTI_M
+  return hipErrorInvalidValue;
+}
+
 // Solve stages whose split has every segment but the last at exactly mmax
 // rows run the unguarded solve (kFlagFull; the sequential-Thomas variant keeps
 // the guarded one).
@@ -819,6 +887,44 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
   case (F):                                                                  \
     return seq ? launch_flags<(F) | kFlagSeq, R>(a.q.mmax, a, s)            \
                : launch_flags<(F), R>(a.q.mmax, a, s);
+
+#define MTS_MULTI_ENTRY MTS_CAT(launch_multi_dl_, MTS_LINE_DIR, MTS_LINE_L)
+#define MTS_MULTI_CASE(F, R)                                                                 \
+  case (F):                                                                                  \
+    return seq ? multi_flags<(F) | kFlagSeq, R>(mmax, full, regular, hargs, nargs, dargs, dwork, \
+                                                nwork, s)                                    \
+               : multi_flags<(F), R>(mmax, full, regular, hargs, nargs, dargs, dwork, nwork, s);
+
+// The solve stages of the split (general and regular tile lists) only.
+hipError_t MTS_MULTI_ENTRY(int flags, bool seq, bool f32, int mmax, bool full, bool regular,
+                           const LineArgs *hargs, int nargs, const LineArgs *dargs,
+                           const int2 *dwork, int nwork, hipStream_t s) {
+  if (f32) {
+    switch (flags) {
+      MTS_MULTI_CASE(kRhsCopy | kFlagSolve, float)
+      MTS_MULTI_CASE(kRhsDouglas | kFlagSolve, float)
+      MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
+      MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+      MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
+      MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+      MTS_MULTI_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
+      default: return hipErrorInvalidValue;
+    }
+  }
+  switch (flags) {
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve, double)
+#if MTS_LINE_DIR == 0
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagPre1, double)
+#endif
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, double)
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, double)
+    MTS_MULTI_CASE(kRhsDouglas | kFlagSolve, double)
+    MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, double)
+    MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, double)
+    MTS_MULTI_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, double)
+    default: return hipErrorInvalidValue;
+  }
+}
 
 hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream_t s) {
   if (f32) {
