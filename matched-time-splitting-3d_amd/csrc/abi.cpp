@@ -210,9 +210,14 @@ struct mts_instance {
   // line stages as their arguments, every other launch as a call on a given
   // stream.
   struct StepOp {
-    int line = 0;  // 1: a split solve stage (dir, flags, a)
+    int line = 0;  // 1: a split solve stage (dir, flags, a); 2: the faces (fa);
+                   // 3: Douglas stage 1 (r1, sp, pairs_too); 0: fn
     int dir = 0, flags = 0;
     mts::LineArgs a;
+    mts::FaceArgs fa;
+    mts::Rhs1Args r1;
+    mts::SpecArgs sp;
+    bool pairs_too = false;
     std::function<hipError_t(hipStream_t)> fn;
   };
   std::vector<StepOp> *rec = nullptr;
@@ -1057,20 +1062,40 @@ static hipError_t line_launch(mts_instance *in, int dir, int flags, const mts::L
   return mts::launch_line(dir, flags, in->f32(), a, st, &in->side);
 }
 
-// A launch other than a line stage: run on the instance's stream (timed when
-// profiling), or recorded.
-extern "C++" {
-template <class F>
-static hipError_t other_op(mts_instance *in, const char *name, F fn) {
+// The boundary faces at time factor e (with the source term's cf): run, or
+// recorded for a launch over several instances.
+static hipError_t faces_op(mts_instance *in, double e, double cf) {
   if (in->rec) {
     mts_instance::StepOp op;
-    op.fn = fn;
+    op.line = 2;
+    op.fa.g = in->geom;
+    op.fa.P = in->phys;
+    op.fa.out = in->field();
+    op.fa.e = e;
+    op.fa.cf = cf;
     in->rec->push_back(std::move(op));
     return hipSuccess;
   }
-  return timed(in, name, [&] { return fn(in->stream); });
+  return timed(in, "faces", [&] {
+    return mts::launch_faces(in->geom, in->phys, in->field(), in->f32(), e, cf, in->stream);
+  });
 }
+// Douglas stage 1: run, or recorded.
+static hipError_t rhs1_op(mts_instance *in, const char *name, const mts::Rhs1Args &r1,
+                          const mts::SpecArgs &sp, bool pairs_too) {
+  if (in->rec) {
+    mts_instance::StepOp op;
+    op.line = 3;
+    op.r1 = r1;
+    op.sp = sp;
+    op.pairs_too = pairs_too;
+    in->rec->push_back(std::move(op));
+    return hipSuccess;
+  }
+  return timed(in, name,
+               [&] { return mts::launch_rhs1(r1, sp, pairs_too, in->f32(), in->stream); });
 }
+
 
 int mts_step(mts_instance *in, double t_next) {
   if (!in) return fail(MTS_E_ARG, "null instance");
@@ -1104,10 +1129,7 @@ int mts_step(mts_instance *in, double t_next) {
     // FP32 form: the whole right-hand side into r first, then the x sweep
     // over r.
     const bool fused = in->douglas_fused && !in->f32();
-    const bool f32 = in->f32();
-    MTS_HIP(other_op(in, fused ? "adi_spec" : "adi_rhs1", [=](hipStream_t s) {
-      return launch_rhs1(r1, sp, !fused, f32, s);
-    }));
+    MTS_HIP(rhs1_op(in, fused ? "adi_spec" : "adi_rhs1", r1, sp, !fused));
     LineArgs a = base_args(in);
     a.e_out = e1;
     a.e_jump = e1;
@@ -1128,12 +1150,7 @@ int mts_step(mts_instance *in, double t_next) {
     a.mib = in->mib[2];
     a.out = in->field();
     MTS_HIP(timed(in, "adi_z", [&] { return line_launch(in, 2, kRhsDouglas | kFlagSolve, a, st); }));
-    const DevGeom geom = in->geom;
-    const PhysConst phys = in->phys;
-    void *field = in->field();
-    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
-      return launch_faces(geom, phys, field, f32, e1, 0.0, s);
-    }));
+    MTS_HIP(faces_op(in, e1, 0.0));
   } else if (in->method == MTS_LOD_IE || in->method == MTS_LOD_CN) {
     const int rhs = in->method == MTS_LOD_CN ? kRhsCN : kRhsCopy;
     LineArgs a = base_args(in);
@@ -1153,13 +1170,7 @@ int mts_step(mts_instance *in, double t_next) {
     MTS_HIP(timed(in, "lod_z", [&] {
       return line_launch(in, 2, rhs | kFlagSolve | kFlagLhsJump | kFlagEpiF, a, st);
     }));
-    const DevGeom geom = in->geom;
-    const PhysConst phys = in->phys;
-    void *field = in->field();
-    const bool f32 = in->f32();
-    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
-      return launch_faces(geom, phys, field, f32, e1, dt, s);
-    }));
+    MTS_HIP(faces_op(in, e1, dt));
   } else {
     const double en = std::exp(-(t_next - dt));
     const double eh = std::exp(-(t_next - 0.5 * dt));
@@ -1189,13 +1200,7 @@ int mts_step(mts_instance *in, double t_next) {
     MTS_HIP(timed(in, "ts_iy", [&] { return line_launch(in, 1, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
     a.mib = in->mib[0];
     MTS_HIP(timed(in, "ts_ix", [&] { return line_launch(in, 0, kRhsCopy | kFlagSolve | kFlagLhsJump, a, st); }));
-    const DevGeom geom = in->geom;
-    const PhysConst phys = in->phys;
-    void *field = in->field();
-    const bool f32 = in->f32();
-    MTS_HIP(other_op(in, "faces", [=](hipStream_t s) {
-      return launch_faces(geom, phys, field, f32, e1, 0.0, s);
-    }));
+    MTS_HIP(faces_op(in, e1, 0.0));
   }
   return MTS_OK;
 }
@@ -1260,7 +1265,34 @@ int mts_batch_run_multi(mts_instance *const *insts, int count, const double *t0,
         }
       for (size_t o = 0; o < nops && rc == MTS_OK; o++) {
         const mts_instance::StepOp &first = ops[act[0]][o];
-        if (first.line) {
+        bool same_kind = true;
+        for (int i : act) same_kind &= ops[i][o].line == first.line;
+        if (!same_kind) {
+          rc = fail(MTS_E_ARG, "instances of one method recorded different steps");
+          break;
+        }
+        if (first.line == 2) {
+          std::vector<mts::FaceArgs> fa;
+          for (int i : act) fa.push_back(ops[i][o].fa);
+          const hipError_t e = mts::launch_faces_multi(fa.data(), (int)fa.size(),
+                                                       insts[act[0]]->f32(), buf, s);
+          if (e != hipSuccess) rc = fail(MTS_E_DEVICE, hipGetErrorString(e));
+        } else if (first.line == 3) {
+          std::vector<mts::Rhs1Args> r1;
+          std::vector<mts::SpecArgs> sp;
+          for (int i : act) {
+            if (ops[i][o].pairs_too != first.pairs_too) {
+              rc = fail(MTS_E_ARG, "instances of one method recorded different steps");
+              break;
+            }
+            r1.push_back(ops[i][o].r1);
+            sp.push_back(ops[i][o].sp);
+          }
+          if (rc) break;
+          const hipError_t e = mts::launch_rhs1_multi(r1.data(), sp.data(), (int)r1.size(),
+                                                      first.pairs_too, insts[act[0]]->f32(), buf, s);
+          if (e != hipSuccess) rc = fail(MTS_E_DEVICE, hipGetErrorString(e));
+        } else if (first.line == 1) {
           args.clear();
           for (int i : act) {
             const mts_instance::StepOp &op = ops[i][o];

@@ -167,16 +167,28 @@ __device__ __forceinline__ double solved_row(const double (&d)[MMAX], int r, int
 // (b mod 8) nper + b / 8, so each XCD walks one contiguous run of z-planes
 // and the y and z neighbours a block reads were fetched into that XCD's L2 by
 // the blocks just before it.
+// The instance of workgroup b of a launch over several instances, whose
+// first workgroups are first[0..count) (ascending, first[0] = 0).
+__device__ __forceinline__ int multi_instance(const int *first, int count, unsigned b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if ((unsigned)first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
 constexpr int kXcds = 8;
 template <typename F>
-__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsigned npairs,
-                                                   unsigned nper, unsigned hp, double inv_hp) {
-  if (blockIdx.x >= nper * kXcds) {
-    const int t = (int)(blockIdx.x - nper * kXcds) * 256 + (int)threadIdx.x;
+__device__ __forceinline__ void rhs1_block(const Rhs1Args &p, const SpecArgs &sp, unsigned npairs,
+                                           unsigned nper, unsigned hp, double inv_hp,
+                                           unsigned blk) {
+  if (blk >= nper * kXcds) {
+    const int t = (int)(blk - nper * kXcds) * 256 + (int)threadIdx.x;
     if (t < sp.nspec) spec_node<F>(sp, t);
     return;
   }
-  const unsigned b = (blockIdx.x % kXcds) * nper + blockIdx.x / kXcds;
+  const unsigned b = (blk % kXcds) * nper + blk / kXcds;
   const unsigned t = b * 256u + threadIdx.x;
   if (t >= npairs) return;
   // t / hp through a double reciprocal, corrected to the exact quotient
@@ -185,16 +197,29 @@ __global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsi
   else if ((row + 1) * hp <= t) row++;
   rhs1_pair<F>(p, row, t - row * hp);
 }
+template <typename F>
+__global__ __launch_bounds__(256) void rhs1_kernel(Rhs1Args p, SpecArgs sp, unsigned npairs,
+                                                   unsigned nper, unsigned hp, double inv_hp) {
+  rhs1_block<F>(p, sp, npairs, nper, hp, inv_hp, blockIdx.x);
+}
+// Douglas stage 1 of several instances in one launch (each instance's
+// workgroups as rhs1_kernel's).
+template <typename F>
+__global__ __launch_bounds__(256) void rhs1_multi_kernel(const Rhs1Multi *args, const int *first,
+                                                         int count) {
+  const int i = multi_instance(first, count, blockIdx.x);
+  const Rhs1Multi &a = args[i];
+  rhs1_block<F>(a.p, a.sp, a.npairs, a.nper, a.hp, a.inv_hp, blockIdx.x - (unsigned)first[i]);
+}
 
 // ------------------------------------------------------- boundary faces
 template <typename R>
-__global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *out, double e,
-                                                    double cf) {
+__device__ __forceinline__ void face_node(const DevGeom &g, const PhysConst &P, R *out, double e,
+                                          double cf, int t) {
   // (32-bit index arithmetic: the six faces hold at most 6 * 1024^2 nodes,
   // and 64-bit division is a long software sequence on the GPU)
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
   const int fz = nx * ny, fy = nx * nz, fx = ny * nz;
-  int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
   int ix, iy, iz;
   if (t < 2 * fz) {
     iz = t < fz ? 0 : nz - 1;
@@ -215,6 +240,20 @@ __global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *o
     return;
   }
   out[node_index(g, ix, iy, iz)] = (R)face_value(P, g, e, cf, ix, iy, iz);
+}
+template <typename R>
+__global__ __launch_bounds__(256) void faces_kernel(DevGeom g, PhysConst P, R *out, double e,
+                                                    double cf) {
+  face_node<R>(g, P, out, e, cf, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+// The boundary faces of several instances in one launch.
+template <typename R>
+__global__ __launch_bounds__(256) void faces_multi_kernel(const FaceArgs *args, const int *first,
+                                                          int count) {
+  const int i = multi_instance(first, count, blockIdx.x);
+  const FaceArgs &a = args[i];
+  face_node<R>(a.g, a.P, static_cast<R *>(a.out), a.e, a.cf,
+               (int)((blockIdx.x - (unsigned)first[i]) * blockDim.x + threadIdx.x));
 }
 
 // ------------------------------------------------------- initialization
@@ -484,12 +523,129 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
 }
 
 MultiBuf::~MultiBuf() {
-  for (int i = 0; i < 2; i++) {
+  for (int i = 0; i < kSlots; i++) {
     if (copied[i]) (void)hipEventSynchronize(copied[i]);
     if (copied[i]) (void)hipEventDestroy(copied[i]);
     if (host[i]) (void)hipHostFree(host[i]);
   }
   if (dev) (void)hipFree(dev);
+}
+
+hipError_t multi_stage(MultiBuf &buf, const void *const *parts, const size_t *sizes, int nparts,
+                       size_t *offs, char **dev, hipStream_t s) {
+  size_t bytes = 0;
+  for (int i = 0; i < nparts; i++) {
+    offs[i] = bytes;
+    bytes = (bytes + sizes[i] + 15) / 16 * 16;
+  }
+  hipError_t e = hipSuccess;
+  if (bytes > buf.host_cap || bytes > buf.dev_cap) {
+    e = hipStreamSynchronize(s);  // (nothing in flight reads the old buffers)
+    if (e != hipSuccess) return e;
+    const size_t cap = bytes * 2;
+    for (int i = 0; i < MultiBuf::kSlots; i++) {
+      if (buf.copied[i]) {
+        e = hipEventSynchronize(buf.copied[i]);
+        if (e != hipSuccess) return e;
+      }
+      if (buf.host[i]) (void)hipHostFree(buf.host[i]);
+      buf.host[i] = nullptr;
+      e = hipHostMalloc(&buf.host[i], cap, hipHostMallocDefault);
+      if (e != hipSuccess) return e;
+      if (!buf.copied[i]) {
+        e = hipEventCreateWithFlags(&buf.copied[i], hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+      }
+    }
+    if (buf.dev) (void)hipFree(buf.dev);
+    buf.dev = nullptr;
+    e = hipMalloc(&buf.dev, cap);
+    if (e != hipSuccess) return e;
+    buf.host_cap = buf.dev_cap = cap;
+  }
+  // the slot's last copy has left it
+  const int sl = buf.slot;
+  buf.slot = (buf.slot + 1) % MultiBuf::kSlots;
+  e = hipEventSynchronize(buf.copied[sl]);
+  if (e != hipSuccess) return e;
+  char *h = static_cast<char *>(buf.host[sl]);
+  for (int i = 0; i < nparts; i++)
+    if (sizes[i]) std::memcpy(h + offs[i], parts[i], sizes[i]);
+  e = hipMemcpyAsync(buf.dev, h, bytes, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipEventRecord(buf.copied[sl], s);
+  *dev = static_cast<char *>(buf.dev);
+  return e;
+}
+
+hipError_t launch_faces_multi(const FaceArgs *args, int count, bool f32, MultiBuf &buf,
+                              hipStream_t s) {
+  if (count < 1) return hipSuccess;
+  std::vector<int> first((size_t)count);
+  long long blocks = 0;
+  for (int i = 0; i < count; i++) {
+    const DevGeom &g = args[i].g;
+    first[(size_t)i] = (int)blocks;
+    const long long total = 2LL * ((long long)g.n[0] * g.n[1] + (long long)g.n[0] * g.n[2] +
+                                   (long long)g.n[1] * g.n[2]);
+    blocks += (total + 255) / 256;
+  }
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const void *parts[2] = {args, first.data()};
+  const size_t sizes[2] = {(size_t)count * sizeof(FaceArgs), first.size() * sizeof(int)};
+  size_t offs[2];
+  char *d = nullptr;
+  hipError_t e = multi_stage(buf, parts, sizes, 2, offs, &d, s);
+  if (e != hipSuccess) return e;
+  const FaceArgs *da = reinterpret_cast<const FaceArgs *>(d + offs[0]);
+  const int *df = reinterpret_cast<const int *>(d + offs[1]);
+  if (f32)
+    hipLaunchKernelGGL(faces_multi_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, da, df,
+                       count);
+  else
+    hipLaunchKernelGGL(faces_multi_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, da, df,
+                       count);
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs1_multi(const Rhs1Args *r1, const SpecArgs *sp, int count, bool pairs_too,
+                             bool f32, MultiBuf &buf, hipStream_t s) {
+  if (count < 1) return hipSuccess;
+  std::vector<Rhs1Multi> m((size_t)count);
+  std::vector<int> first((size_t)count);
+  long long blocks = 0;
+  for (int i = 0; i < count; i++) {
+    const Rhs1Args &a = r1[i];
+    if (a.g.px % 2 != 0 || sp[i].nspec < 0) return hipErrorInvalidValue;
+    const unsigned hp = (unsigned)(a.g.px / 2);
+    const size_t pairs = pairs_too ? (size_t)hp * a.g.n[1] * a.g.n[2] : 0;
+    if (pairs > 0xffffffffu - 256u * kXcds) return hipErrorInvalidValue;
+    Rhs1Multi &q = m[(size_t)i];
+    q.p = a;
+    q.sp = sp[i];
+    q.npairs = (unsigned)pairs;
+    q.nper = (unsigned)(((pairs + 255) / 256 + kXcds - 1) / kXcds);
+    q.hp = hp;
+    q.inv_hp = 1.0 / hp;
+    first[(size_t)i] = (int)blocks;
+    blocks += (long long)q.nper * kXcds + (sp[i].nspec + 255) / 256;
+  }
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+  const void *parts[2] = {m.data(), first.data()};
+  const size_t sizes[2] = {m.size() * sizeof(Rhs1Multi), first.size() * sizeof(int)};
+  size_t offs[2];
+  char *d = nullptr;
+  hipError_t e = multi_stage(buf, parts, sizes, 2, offs, &d, s);
+  if (e != hipSuccess) return e;
+  const Rhs1Multi *da = reinterpret_cast<const Rhs1Multi *>(d + offs[0]);
+  const int *df = reinterpret_cast<const int *>(d + offs[1]);
+  if (f32)
+    hipLaunchKernelGGL(rhs1_multi_kernel<float>, dim3((unsigned)blocks), dim3(256), 0, s, da, df,
+                       count);
+  else
+    hipLaunchKernelGGL(rhs1_multi_kernel<double>, dim3((unsigned)blocks), dim3(256), 0, s, da, df,
+                       count);
+  return hipGetLastError();
 }
 
 hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args, int count,
@@ -504,7 +660,8 @@ hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args,
   struct Group {
     Key k;
     std::vector<LineArgs> args;
-    std::vector<int2> work;
+    std::vector<int> first;  // each slot's first workgroup
+    int blocks = 0;
   };
   std::vector<Group> groups;
   for (int i = 0; i < count; i++) {
@@ -523,64 +680,32 @@ hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args,
       Key k{a.q.L, a.q.mmax, a.q.m == a.q.mmax ? 1 : 0, a.mib.seq != nullptr ? 1 : 0, regular};
       size_t gi = 0;
       while (gi < groups.size() && !(groups[gi].k == k)) gi++;
-      if (gi == groups.size()) groups.push_back(Group{k, {}, {}});
+      if (gi == groups.size()) groups.push_back(Group{k, {}, {}, 0});
       Group &g = groups[gi];
-      const int slot = (int)g.args.size();
       g.args.push_back(a);
-      for (int t = 0; t < nt; t++) g.work.push_back(make_int2(slot, t));
+      g.first.push_back(g.blocks);
+      g.blocks += nt;
     }
   }
   if (groups.empty()) return hipSuccess;
   // one staging image: per group its arguments, then its work entries
-  auto up16 = [](size_t b) { return (b + 15) / 16 * 16; };
-  std::vector<size_t> off_args(groups.size()), off_work(groups.size());
-  size_t bytes = 0;
-  for (size_t gi = 0; gi < groups.size(); gi++) {
-    off_args[gi] = bytes;
-    bytes = up16(bytes + groups[gi].args.size() * sizeof(LineArgs));
-    off_work[gi] = bytes;
-    bytes = up16(bytes + groups[gi].work.size() * sizeof(int2));
+  std::vector<const void *> parts;
+  std::vector<size_t> sizes;
+  for (const Group &g : groups) {
+    parts.push_back(g.args.data());
+    sizes.push_back(g.args.size() * sizeof(LineArgs));
+    parts.push_back(g.first.data());
+    sizes.push_back(g.first.size() * sizeof(int));
   }
-  hipError_t e = hipSuccess;
-  if (bytes > buf.host_cap || bytes > buf.dev_cap) {
-    e = hipStreamSynchronize(s);  // (nothing in flight reads the old buffers)
-    if (e != hipSuccess) return e;
-    const size_t cap = bytes * 2;
-    for (int i = 0; i < 2; i++) {
-      if (buf.host[i]) (void)hipHostFree(buf.host[i]);
-      buf.host[i] = nullptr;
-      e = hipHostMalloc(&buf.host[i], cap, hipHostMallocDefault);
-      if (e != hipSuccess) return e;
-      if (!buf.copied[i]) {
-        e = hipEventCreateWithFlags(&buf.copied[i], hipEventDisableTiming);
-        if (e != hipSuccess) return e;
-      }
-    }
-    if (buf.dev) (void)hipFree(buf.dev);
-    buf.dev = nullptr;
-    e = hipMalloc(&buf.dev, cap);
-    if (e != hipSuccess) return e;
-    buf.host_cap = buf.dev_cap = cap;
-  }
-  // the slot's last copy has left it
-  const int sl = buf.slot;
-  buf.slot ^= 1;
-  e = hipEventSynchronize(buf.copied[sl]);
+  std::vector<size_t> offs(parts.size());
+  char *d = nullptr;
+  hipError_t e = multi_stage(buf, parts.data(), sizes.data(), (int)parts.size(), offs.data(), &d, s);
   if (e != hipSuccess) return e;
-  char *h = static_cast<char *>(buf.host[sl]);
-  for (size_t gi = 0; gi < groups.size(); gi++) {
-    std::memcpy(h + off_args[gi], groups[gi].args.data(), groups[gi].args.size() * sizeof(LineArgs));
-    std::memcpy(h + off_work[gi], groups[gi].work.data(), groups[gi].work.size() * sizeof(int2));
-  }
-  e = hipMemcpyAsync(buf.dev, h, bytes, hipMemcpyHostToDevice, s);
-  if (e == hipSuccess) e = hipEventRecord(buf.copied[sl], s);
-  if (e != hipSuccess) return e;
-  char *d = static_cast<char *>(buf.dev);
   for (size_t gi = 0; gi < groups.size() && e == hipSuccess; gi++) {
     const Group &g = groups[gi];
-    const LineArgs *da = reinterpret_cast<const LineArgs *>(d + off_args[gi]);
-    const int2 *dw = reinterpret_cast<const int2 *>(d + off_work[gi]);
-    const int na = (int)g.args.size(), nw = (int)g.work.size();
+    const LineArgs *da = reinterpret_cast<const LineArgs *>(d + offs[2 * gi]);
+    const int *dw = reinterpret_cast<const int *>(d + offs[2 * gi + 1]);
+    const int na = (int)g.args.size(), nw = g.blocks;
     const bool full = g.k.full != 0, seq = g.k.seq != 0, reg = g.k.regular != 0;
 #define MTS_ML(D, LL)                                                                          \
   if (dir == D && g.k.L == LL) {                                                               \

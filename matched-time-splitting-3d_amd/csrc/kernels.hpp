@@ -36,19 +36,21 @@ MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
 #define MTS_DECL_MULTI(D, LL)                                                                 \
   hipError_t launch_multi_dl_##D##_##LL(int flags, bool seq, bool f32, int mmax, bool full,     \
                                         bool regular, const LineArgs *hargs, int nargs,         \
-                                        const LineArgs *dargs, const int2 *dwork, int nwork,    \
+                                        const LineArgs *dargs, const int *dfirst, int nwork,    \
                                         hipStream_t s);
 MTS_DECL_MULTI(0, 16) MTS_DECL_MULTI(0, 32) MTS_DECL_MULTI(0, 64)
 MTS_DECL_MULTI(1, 16) MTS_DECL_MULTI(1, 32) MTS_DECL_MULTI(1, 64)
 MTS_DECL_MULTI(2, 16) MTS_DECL_MULTI(2, 32) MTS_DECL_MULTI(2, 64)
 #undef MTS_DECL_MULTI
 
-// Staging of a multi-instance line stage's arguments and work lists: pinned
-// host memory (two slots, each reused once the copy out of it is done) and
-// one device buffer, grown as needed.  Used on one stream.
+// Staging of a multi-instance launch's arguments and work lists: pinned host
+// memory (kSlots slots used in turn, each reused once the copy out of it is
+// done, so the host runs up to kSlots launches ahead) and one device buffer,
+// grown as needed.  Used on one stream.
 struct MultiBuf {
-  void *host[2] = {nullptr, nullptr};
-  hipEvent_t copied[2] = {nullptr, nullptr};
+  static constexpr int kSlots = 32;
+  void *host[kSlots] = {};
+  hipEvent_t copied[kSlots] = {};
   size_t host_cap = 0;
   int slot = 0;
   void *dev = nullptr;
@@ -58,6 +60,31 @@ struct MultiBuf {
   MultiBuf &operator=(const MultiBuf &) = delete;
   ~MultiBuf();
 };
+// Staging of host bytes for a launch over several instances: parts laid out
+// one after the other at 16-byte boundaries (offs), copied into the device
+// buffer on s; returns the device base address in *dev.
+hipError_t multi_stage(MultiBuf &buf, const void *const *parts, const size_t *sizes, int nparts,
+                       size_t *offs, char **dev, hipStream_t s);
+
+// The boundary faces (launch_faces) and Douglas stage 1 (launch_rhs1) of
+// several instances, one launch each.
+struct FaceArgs {
+  DevGeom g;
+  PhysConst P;
+  void *out;
+  double e, cf;
+};
+struct Rhs1Multi {
+  Rhs1Args p;
+  SpecArgs sp;
+  unsigned npairs, nper, hp;
+  double inv_hp;
+};
+hipError_t launch_faces_multi(const FaceArgs *args, int count, bool f32, MultiBuf &buf,
+                              hipStream_t s);
+hipError_t launch_rhs1_multi(const Rhs1Args *r1, const SpecArgs *sp, int count, bool pairs_too,
+                             bool f32, MultiBuf &buf, hipStream_t s);
+
 // One solve stage (dir, flags) of count instances in as few launches as
 // their kernels allow: the general and the regular tiles of every instance
 // whose split, register bound, full split and sequential lines match go into

@@ -568,13 +568,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) voi
   tile_coords<DIR, tile_lines(L), R>(p.split ? p.tiles[1] : nullptr, bx, by);
   line_body<DIR, L, MMAX, FLAGS, R>(p, bx, by);
 }
+// The slot of workgroup b in a launch over several instances whose first
+// workgroups are first[0..count) (ascending, first[0] = 0; a slot without
+// workgroups shares its start with the next one), and b's tile in it.
+__device__ __forceinline__ int2 multi_slot(const int *first, int count, unsigned b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) / 2;
+    if ((unsigned)first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return make_int2(lo, (int)(b - (unsigned)first[lo]));
+}
 // The general tiles of several instances' line stage in one launch:
-// workgroup b solves tile work[b].y of the list of instance work[b].x, whose
-// stage arguments are args[work[b].x] (in device memory).
+// workgroup b solves tile y of the list of the instance in slot x
+// (multi_slot), whose stage arguments are args[x] (in device memory).
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(2))) void line_kernel_multi(
-    const LineArgs *args, const int2 *work) {
-  const int2 w = work[blockIdx.x];
+    const LineArgs *args, const int *first, int count) {
+  const int2 w = multi_slot(first, count, blockIdx.x);
   const LineArgs &p = args[w.x];
   const int e = p.tiles[1][w.y];
   line_body<DIR, L, MMAX, FLAGS, R>(p, tile_bx(e), tile_k(e) - 1);
@@ -770,8 +782,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) voi
 // line_kernel_multi).
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(4))) void line_kernel_reg_multi(
-    const LineArgs *args, const int2 *work) {
-  const int2 w = work[blockIdx.x];
+    const LineArgs *args, const int *first, int count) {
+  const int2 w = multi_slot(first, count, blockIdx.x);
   const LineArgs &p = args[w.x];
   reg_body<DIR, L, MMAX, FLAGS, R>(p, p.tiles[0][w.y]);
 }
@@ -815,11 +827,11 @@ hipError_t launch_line_t(const LineArgs &a, hipStream_t st) {
 
 // One group of a multi-instance line stage: the general tiles (regular
 // false) or the regular tiles of the instances whose stage arguments are
-// hargs[0..nargs) (host copies, for the LDS size) and dargs (device), one
-// workgroup per entry of dwork[0..nwork).
+// hargs[0..nargs) (host copies, for the LDS size) and dargs (device), nwork
+// workgroups, those of slot i from dfirst[i].
 template <int DIR, int L, int MMAX, int FLAGS, typename R>
 hipError_t launch_multi_t(bool regular, const LineArgs *hargs, int nargs, const LineArgs *dargs,
-                          const int2 *dwork, int nwork, hipStream_t st) {
+                          const int *dfirst, int nwork, hipStream_t st) {
   using LL = LdsLayout<DIR, L, MMAX, FLAGS, R>;
   constexpr bool FULL = (FLAGS & kFlagFull) != 0;
   size_t shm = 0;
@@ -833,15 +845,15 @@ hipError_t launch_multi_t(bool regular, const LineArgs *hargs, int nargs, const 
   if (nwork <= 0) return hipSuccess;
   if (regular)
     hipLaunchKernelGGL((line_kernel_reg_multi<DIR, L, MMAX, FLAGS, R>), dim3(nwork), dim3(kBlock),
-                       shm, st, dargs, dwork);
+                       shm, st, dargs, dfirst, nargs);
   else
     hipLaunchKernelGGL((line_kernel_multi<DIR, L, MMAX, FLAGS, R>), dim3(nwork), dim3(kBlock), shm,
-                       st, dargs, dwork);
+                       st, dargs, dfirst, nargs);
   return hipGetLastError();
 }
 template <int FLAGS, typename R>
 hipError_t multi_flags(int mmax, bool full, bool regular, const LineArgs *hargs, int nargs,
-                       const LineArgs *dargs, const int2 *dwork, int nwork, hipStream_t st) {
+                       const LineArgs *dargs, const int *dwork, int nwork, hipStream_t st) {
   constexpr int D = MTS_LINE_DIR, LL = MTS_LINE_L;
   constexpr bool CAN_FULL = (FLAGS & kFlagSolve) && !(FLAGS & kFlagSeq);
   constexpr int FF = CAN_FULL ? (FLAGS | kFlagFull) : FLAGS;
@@ -888,6 +900,7 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
     return seq ? launch_flags<(F) | kFlagSeq, R>(a.q.mmax, a, s)            \
                : launch_flags<(F), R>(a.q.mmax, a, s);
 
+#if MTS_LINE_MULTI
 #define MTS_MULTI_ENTRY MTS_CAT(launch_multi_dl_, MTS_LINE_DIR, MTS_LINE_L)
 #define MTS_MULTI_CASE(F, R)                                                                 \
   case (F):                                                                                  \
@@ -898,7 +911,7 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
 // The solve stages of the split (general and regular tile lists) only.
 hipError_t MTS_MULTI_ENTRY(int flags, bool seq, bool f32, int mmax, bool full, bool regular,
                            const LineArgs *hargs, int nargs, const LineArgs *dargs,
-                           const int2 *dwork, int nwork, hipStream_t s) {
+                           const int *dwork, int nwork, hipStream_t s) {
   if (f32) {
     switch (flags) {
       MTS_MULTI_CASE(kRhsCopy | kFlagSolve, float)
@@ -926,6 +939,7 @@ hipError_t MTS_MULTI_ENTRY(int flags, bool seq, bool f32, int mmax, bool full, b
   }
 }
 
+#else
 hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream_t s) {
   if (f32) {
     // the FP32 form: LOD, TS and Douglas-ADI (its stage 1 outside the x sweep)
@@ -956,5 +970,7 @@ hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream
     default: return hipErrorInvalidValue;
   }
 }
+
+#endif
 
 }  // namespace mts
