@@ -1,7 +1,8 @@
 /*
  * ORACLE (test infrastructure only): a CPU restatement in plain C of the
  * reference's per-time-step path.  Only tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg load it, as the checker; the product never does.
+ * bench.py's cpu_baseline leg load it, to check results against; the product
+ * never does.
  *
  * It follows the reference's algorithm line by line (see mts_oracle.c for the
  * file:line of each function) and takes the interface crossings with their

@@ -3,7 +3,7 @@
 ctypes wrapper of oracle/_build/libmts_oracle.so, the plain-C restatement of
 the reference's per-step path, and of oracle/_ref/mts_ref, the reference
 solver built from its own sources.  Only tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg use this module, as the checker.
+bench.py's cpu_baseline leg use this module, to check results against.
 """
 from __future__ import annotations
 

@@ -46,17 +46,17 @@ for _m in ("adi", "lod-ie", "lod-cn", "ts"):
         CASES.append(dict(name=f"sphere16_{_m}_mib{_mib}", method=_m, mib=_mib, n=16, dt=0.125,
                           T=0.5, surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1)))
 CASES += [
-    # Douglas-ADI + MIB-L1, 33^3 nodes, sphere r=0.5 (pulled off the grid), 16 steps
+    # the 33^3 Douglas-ADI workload: MIB-L1, sphere r=0.5 (pulled off the grid), 16 steps
     dict(name="cfg0_adi_mib1", method="adi", mib=1, n=32, dt=0.0625, T=1.0,
          surface=("ellipsoid", 1, 1, 1, R0), beta=("const", 10, 1)),
-    # configs[1] geometry and physics at 33^3 (the reference cannot set up 257^3: its
-    # tangential-jump diagnostics exit for near-tangent crossings)
+    # the 257^3 LOD-CN workload's geometry and physics at 33^3 (the reference cannot set
+    # up 257^3: its tangential-jump diagnostics exit for near-tangent crossings)
     dict(name="cfg1like32_lodcn_mib2", method="lod-cn", mib=2, n=32, dt=1 / 16, T=0.5,
          surface=("ellipsoid", 0.7, 0.5, 0.4, E1), beta=("sincos", 10, 1)),
-    # configs[2] physics (beta 1000/1, bumpy sphere) at 27^3
+    # the 513^3 TS workload's physics (beta 1000/1, bumpy sphere) at 27^3
     dict(name="bumpy26_ts_mib2", method="ts", mib=2, n=26, dt=1 / 13, T=0.25,
          surface=("bumpy", 0.5 * (1 - 2 ** -20), 0.1, 5, 3), beta=("const", 1000, 1)),
-    # configs[3]/[4] geometry family: torus, strong contrast either way
+    # the batch and serving mixes' geometry family: torus, strong contrast either way
     dict(name="torus22_lodcn_mib2", method="lod-cn", mib=2, n=22, dt=1 / 11, T=0.5,
          surface=("torus", 0.5, 0.2), beta=("const", 0.01, 1)),
     dict(name="torus22_lodie_mib1", method="lod-ie", mib=1, n=22, dt=1 / 11, T=0.5,
@@ -64,7 +64,7 @@ CASES += [
     dict(name="ellipsoid20_adi_mib2", method="adi", mib=2, n=20, dt=0.1, T=0.5,
          surface=("ellipsoid", 0.8, 0.6, 0.5, E1), beta=("const", 100, 1)),
     # Douglas-ADI on interfaces with corner pairs (32 corner crossings along z on
-    # the torus, 8 along y and 8 along z on the configs[1] ellipsoid)
+    # the torus, 8 along y and 8 along z on the 257^3 LOD-CN workload's ellipsoid)
     dict(name="torus22_adi_mib1", method="adi", mib=1, n=22, dt=1 / 11, T=0.5,
          surface=("torus", 0.5, 0.17), beta=("const", 100, 1)),
     dict(name="torus22_adi_mib2", method="adi", mib=2, n=22, dt=1 / 11, T=0.5,
