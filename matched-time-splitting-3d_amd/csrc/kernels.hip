@@ -490,7 +490,9 @@ hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &args, hipSt
     if (tile_lines(a.q.L) * a.g.n[dir] > kTileElsPerThread * kBlockThreads) return hipErrorInvalidValue;
     const bool seq = (flags & kFlagSolve) && a.mib.seq != nullptr;
 #define MTS_DL(D, LL) \
-  if (dir == D && a.q.L == LL) return launch_line_dl_##D##_##LL(flags, seq, f32, a, s);
+  if (dir == D && a.q.L == LL)                                                           \
+    return f32 ? launch_line_dl_##D##_##LL##_1(flags, seq, f32, a, s)                    \
+               : launch_line_dl_##D##_##LL##_0(flags, seq, f32, a, s);
     MTS_DL(0, 16) MTS_DL(0, 32) MTS_DL(0, 64)
     MTS_DL(1, 16) MTS_DL(1, 32) MTS_DL(1, 64)
     MTS_DL(2, 16) MTS_DL(2, 32) MTS_DL(2, 64)
@@ -709,8 +711,10 @@ hipError_t launch_line_multi(int dir, int flags, bool f32, const LineArgs *args,
     const bool full = g.k.full != 0, seq = g.k.seq != 0, reg = g.k.regular != 0;
 #define MTS_ML(D, LL)                                                                          \
   if (dir == D && g.k.L == LL) {                                                               \
-    e = launch_multi_dl_##D##_##LL(flags, seq, f32, g.k.mmax, full, reg, g.args.data(), na, da, \
-                                   dw, nw, s);                                                 \
+    e = f32 ? launch_multi_dl_##D##_##LL##_1(flags, seq, f32, g.k.mmax, full, reg,            \
+                                             g.args.data(), na, da, dw, nw, s)                 \
+            : launch_multi_dl_##D##_##LL##_0(flags, seq, f32, g.k.mmax, full, reg,            \
+                                             g.args.data(), na, da, dw, nw, s);                \
     continue;                                                                                  \
   }
     MTS_ML(0, 16) MTS_ML(0, 32) MTS_ML(0, 64)

@@ -25,23 +25,28 @@ struct SideStream {
 hipError_t launch_line(int dir, int flags, bool f32, const LineArgs &a, hipStream_t s,
                        const SideStream *side = nullptr);
 
-// Per direction and lane-group size (line_kernels.hip); a.q already filled.
-#define MTS_DECL_DL(D, LL)                                                              \
-  hipError_t launch_line_dl_##D##_##LL(int flags, bool seq, bool f32, const LineArgs &a, \
-                                       hipStream_t s);
+// Per direction, lane-group size and field type (line_kernels.hip: _0 FP64,
+// _1 FP32); a.q already filled.
+#define MTS_DECL_DL(D, LL)                                                                \
+  hipError_t launch_line_dl_##D##_##LL##_0(int flags, bool seq, bool f32, const LineArgs &a, \
+                                           hipStream_t s);                                  \
+  hipError_t launch_line_dl_##D##_##LL##_1(int flags, bool seq, bool f32, const LineArgs &a, \
+                                           hipStream_t s);
 MTS_DECL_DL(0, 16) MTS_DECL_DL(0, 32) MTS_DECL_DL(0, 64)
 MTS_DECL_DL(1, 16) MTS_DECL_DL(1, 32) MTS_DECL_DL(1, 64)
 MTS_DECL_DL(2, 16) MTS_DECL_DL(2, 32) MTS_DECL_DL(2, 64)
 #undef MTS_DECL_DL
-#define MTS_DECL_MULTI(D, LL)                                                                 \
-  hipError_t launch_multi_dl_##D##_##LL(int flags, bool seq, bool f32, int mmax, bool full,     \
-                                        bool regular, const LineArgs *hargs, int nargs,         \
-                                        const LineArgs *dargs, const int *dfirst, int nwork,    \
-                                        hipStream_t s);
+#define MTS_DECL_MULTI1(D, LL, P)                                                             \
+  hipError_t launch_multi_dl_##D##_##LL##_##P(int flags, bool seq, bool f32, int mmax,          \
+                                              bool full, bool regular, const LineArgs *hargs,   \
+                                              int nargs, const LineArgs *dargs,                 \
+                                              const int *dfirst, int nwork, hipStream_t s);
+#define MTS_DECL_MULTI(D, LL) MTS_DECL_MULTI1(D, LL, 0) MTS_DECL_MULTI1(D, LL, 1)
 MTS_DECL_MULTI(0, 16) MTS_DECL_MULTI(0, 32) MTS_DECL_MULTI(0, 64)
 MTS_DECL_MULTI(1, 16) MTS_DECL_MULTI(1, 32) MTS_DECL_MULTI(1, 64)
 MTS_DECL_MULTI(2, 16) MTS_DECL_MULTI(2, 32) MTS_DECL_MULTI(2, 64)
 #undef MTS_DECL_MULTI
+#undef MTS_DECL_MULTI1
 
 // Staging of a multi-instance launch's arguments and work lists: pinned host
 // memory (kSlots slots used in turn, each reused once the copy out of it is

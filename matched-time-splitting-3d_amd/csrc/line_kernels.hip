@@ -889,9 +889,11 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
 
 }  // namespace
 
-#define MTS_CAT2(a, b, c) a##b##_##c
-#define MTS_CAT(a, b, c) MTS_CAT2(a, b, c)
-#define MTS_ENTRY MTS_CAT(launch_line_dl_, MTS_LINE_DIR, MTS_LINE_L)
+// Each object holds one direction, lane-group size and field type (its entry
+// points end in _0 for FP64, _1 for FP32).
+#define MTS_CAT2(a, b, c, d) a##b##_##c##_##d
+#define MTS_CAT(a, b, c, d) MTS_CAT2(a, b, c, d)
+#define MTS_ENTRY MTS_CAT(launch_line_dl_, MTS_LINE_DIR, MTS_LINE_L, MTS_LINE_F32)
 
 // The sequential-solve variant is compiled separately so that its register
 // needs do not lower the occupancy of the common case (no such line).
@@ -901,7 +903,7 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
                : launch_flags<(F), R>(a.q.mmax, a, s);
 
 #if MTS_LINE_MULTI
-#define MTS_MULTI_ENTRY MTS_CAT(launch_multi_dl_, MTS_LINE_DIR, MTS_LINE_L)
+#define MTS_MULTI_ENTRY MTS_CAT(launch_multi_dl_, MTS_LINE_DIR, MTS_LINE_L, MTS_LINE_F32)
 #define MTS_MULTI_CASE(F, R)                                                                 \
   case (F):                                                                                  \
     return seq ? multi_flags<(F) | kFlagSeq, R>(mmax, full, regular, hargs, nargs, dargs, dwork, \
@@ -912,18 +914,19 @@ hipError_t launch_flags(int mmax, const LineArgs &a, hipStream_t st) {
 hipError_t MTS_MULTI_ENTRY(int flags, bool seq, bool f32, int mmax, bool full, bool regular,
                            const LineArgs *hargs, int nargs, const LineArgs *dargs,
                            const int *dwork, int nwork, hipStream_t s) {
-  if (f32) {
-    switch (flags) {
-      MTS_MULTI_CASE(kRhsCopy | kFlagSolve, float)
-      MTS_MULTI_CASE(kRhsDouglas | kFlagSolve, float)
-      MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
-      MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
-      MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
-      MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
-      MTS_MULTI_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
-      default: return hipErrorInvalidValue;
-    }
+  if (f32 != (MTS_LINE_F32 != 0)) return hipErrorInvalidValue;
+#if MTS_LINE_F32
+  switch (flags) {
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve, float)
+    MTS_MULTI_CASE(kRhsDouglas | kFlagSolve, float)
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
+    MTS_MULTI_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+    MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
+    MTS_MULTI_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+    MTS_MULTI_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
+    default: return hipErrorInvalidValue;
   }
+#else
   switch (flags) {
     MTS_MULTI_CASE(kRhsCopy | kFlagSolve, double)
 #if MTS_LINE_DIR == 0
@@ -937,24 +940,26 @@ hipError_t MTS_MULTI_ENTRY(int flags, bool seq, bool f32, int mmax, bool full, b
     MTS_MULTI_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, double)
     default: return hipErrorInvalidValue;
   }
+#endif
 }
 
 #else
 hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream_t s) {
-  if (f32) {
-    // the FP32 form: LOD, TS and Douglas-ADI (its stage 1 outside the x sweep)
-    switch (flags) {
-      MTS_LINE_CASE(kRhsCopy | kFlagSolve, float)
-      MTS_LINE_CASE(kRhsDouglas | kFlagSolve, float)
-      MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
-      MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
-      MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
-      MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
-      MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
-      case kRhsCN: return launch_flags<kRhsCN, float>(a.q.mmax, a, s);
-      default: return hipErrorInvalidValue;
-    }
+  if (f32 != (MTS_LINE_F32 != 0)) return hipErrorInvalidValue;
+#if MTS_LINE_F32
+  // the FP32 form: LOD, TS and Douglas-ADI (its stage 1 outside the x sweep)
+  switch (flags) {
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve, float)
+    MTS_LINE_CASE(kRhsDouglas | kFlagSolve, float)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump, float)
+    MTS_LINE_CASE(kRhsCopy | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump, float)
+    MTS_LINE_CASE(kRhsCN | kFlagSolve | kFlagLhsJump | kFlagEpiF, float)
+    MTS_LINE_CASE(kRhsTSZ | kFlagSolve | kFlagLhsJump, float)
+    case kRhsCN: return launch_flags<kRhsCN, float>(a.q.mmax, a, s);
+    default: return hipErrorInvalidValue;
   }
+#else
   switch (flags) {
     MTS_LINE_CASE(kRhsCopy | kFlagSolve, double)
 #if MTS_LINE_DIR == 0
@@ -969,6 +974,7 @@ hipError_t MTS_ENTRY(int flags, bool seq, bool f32, const LineArgs &a, hipStream
     case kRhsCN: return launch_flags<kRhsCN, double>(a.q.mmax, a, s);
     default: return hipErrorInvalidValue;
   }
+#endif
 }
 
 #endif
