@@ -432,6 +432,8 @@ __device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
   constexpr bool SOLVE = (FLAGS & kFlagSolve) != 0;
   constexpr bool HAS_S = (RHS == kRhsDouglas);
   constexpr bool TWO = HAS_S || !SOLVE;  // second LDS tile: S, or the output of explicit stages
+  // streaming field accesses in the FP64 Douglas stages (as line_kernel_reg)
+  constexpr bool NT = (HAS_S || (DIR == 0 && (FLAGS & kFlagPre1) != 0)) && sizeof(R) == sizeof(double);
   constexpr bool SEQ = SOLVE && (FLAGS & kFlagSeq) != 0;
   constexpr bool FULL = (FLAGS & kFlagFull) != 0;
   constexpr int LPW = 64 / L;          // lines per wavefront
@@ -501,8 +503,13 @@ __device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
       va[c].x = va[c].y = R(0);
       vb[c].x = vb[c].y = R(0);
       if (tp[c].ld0 || tp[c].ld1) {  // (ld1 implies the first element exists)
-        va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].gi);
-        if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].gi);
+        if constexpr (NT) {
+          va[c] = nt_load2(reinterpret_cast<const V2 *>(fA + tp[c].gi));
+          if (HAS_S) vb[c] = nt_load2(reinterpret_cast<const V2 *>(fS + tp[c].gi));
+        } else {
+          va[c] = *reinterpret_cast<const V2 *>(fA + tp[c].gi);
+          if (HAS_S) vb[c] = *reinterpret_cast<const V2 *>(fS + tp[c].gi);
+        }
       }
     }
     const TileWalk<DIR, T> walk_st = fresh_walk<DIR, T>(p, j0, k, tid);
@@ -578,7 +585,8 @@ __device__ __forceinline__ void line_body(const LineArgs &p, int bx, int by) {
     for (int c = 0; c < kChunk; c++) {
       const WalkPair &t = tp[c];
       if (t.w0 && t.w1) {
-        *reinterpret_cast<V2 *>(fO + t.gi) = vo[c];
+        if constexpr (NT) nt_store2(reinterpret_cast<V2 *>(fO + t.gi), vo[c]);
+        else *reinterpret_cast<V2 *>(fO + t.gi) = vo[c];
       } else {
         if (t.w0) fO[t.gi] = vo[c].x;
         if (t.w1) fO[t.gi + 1] = vo[c].y;
